@@ -1,0 +1,322 @@
+"""Benchmark of the Parallax primitive hot path on one MI355X.
+
+Default workload (the plain per-call configuration): one step calls, through the C ABI
+of parallax/runtime.h with device-resident (HBM) operands, exactly as the reference's
+funnels call it:
+    reduce  2^28 float32 normal(0,1)          reduce  2^28 int64 uniform [-2^20, 2^20)
+    inclusive scan 2^28 uint32 in [0,16)       exclusive scan 2^28 uint32 in [0,16)
+    sort    2^26 uint32 (1% duplicates)        sort    2^25 float64 (1% duplicates)
+    copy_if 2^28 int32, predicate x % 2 == 0   (all from seed 1)
+`value` = algorithmic bytes of the step / wall time of the step (GB/s); algorithmic
+bytes: reduce n*s, scans 2*n*s, sort 2*n*s, copy_if n*s + kept*s.  Each call's device
+time is also measured with HIP events on the launcher's stream (pxr_prof_*), giving
+per-primitive GB/s and the roofline of the dominant primitive.  Results are checked
+against the oracle before timing.
+
+Other workloads (--workload batched | served) run the batched and served forms.
+"""
+import argparse
+import ctypes
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "parallax-runtime_amd"))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "GB/s and % HBM peak per primitive; batched/served req/s + p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ plain suite
+def plain_suite(args):
+    from parallax_amd import lib, make_desc, descriptor_words, consts as C, datagen
+    from parallax_amd.device import DeviceBuffer, stats
+    from parallax_amd.dtypes import PX_F32, PX_F64, PX_I32, PX_I64, PX_U32
+
+    scale = args.scale
+    t0 = time.time()
+    X = datagen.plain_suite_inputs(seed=1, scale=scale)
+    log(f"[bench] inputs generated in {time.time() - t0:.1f}s")
+
+    def load(d):
+        w, n = descriptor_words(d)
+        k = lib.parallax_kernel_load(w, n)
+        if not k:
+            raise RuntimeError("kernel load failed (no GPU?)")
+        return k
+
+    K = {
+        "red_f32": load(make_desc(C.OP_REDUCE, PX_F32)),
+        "red_i64": load(make_desc(C.OP_REDUCE, PX_I64)),
+        "scan": load(make_desc(C.OP_SCAN, PX_U32)),
+        "add": load(make_desc(C.OP_SCAN_ADD, PX_U32)),
+        "shift": load(make_desc(C.OP_SCAN_SHIFT, PX_U32)),
+        "sort_u32": load(make_desc(C.OP_SORT, PX_U32)),
+        "sort_f64": load(make_desc(C.OP_SORT, PX_F64)),
+        "flags": load(make_desc(C.OP_FLAGS, PX_I32, fn=C.PRED_MOD_EQ, c0=2, c1=0)),
+        "scan_i32": load(make_desc(C.OP_SCAN, PX_I32)),
+        "add_i32": load(make_desc(C.OP_SCAN_ADD, PX_I32)),
+        "scatter": load(make_desc(C.OP_SCATTER, PX_I32, aux=C.SCATTER_COMPACT)),
+    }
+    nsteps = args.warmup + args.steps + 1  # +1: the verification call
+    t0 = time.time()
+    D = {
+        "red_f32": DeviceBuffer.from_array(X["reduce_f32"]),
+        "red_i64": DeviceBuffer.from_array(X["reduce_i64"]),
+        "scan_in": DeviceBuffer.from_array(X["scan_u32"]),
+        "scan_inplace": DeviceBuffer.from_array(X["scan_u32"]),
+        "scan_out": DeviceBuffer(X["scan_u32"].nbytes),
+        "cif_in": DeviceBuffer.from_array(X["copy_if_i32"]),
+        "cif_out": DeviceBuffer(X["copy_if_i32"].nbytes),
+        "res": DeviceBuffer(64),
+    }
+    sort_u32 = [DeviceBuffer.from_array(X["sort_u32"]) for _ in range(nsteps)]
+    sort_f64 = [DeviceBuffer.from_array(X["sort_f64"]) for _ in range(nsteps)]
+    lib.pxd_device_sync()
+    log(f"[bench] inputs resident in HBM in {time.time() - t0:.1f}s")
+
+    n28 = X["reduce_f32"].size
+    n26 = X["sort_u32"].size
+    n25 = X["sort_f64"].size
+    init = np.zeros(1, dtype=np.uint32)
+    kept_box = [0]
+
+    def step(i):
+        r = D["res"].ptr
+        lib.parallax_reduce(K["red_f32"], D["red_f32"].ptr, n28, 4, r)
+        lib.parallax_reduce(K["red_i64"], D["red_i64"].ptr, n28, 8, r + 8)
+        lib.parallax_scan(K["scan"], K["add"], D["scan_inplace"].ptr, n28, 4)
+        lib.parallax_exclusive_scan(K["scan"], K["add"], K["shift"], D["scan_in"].ptr,
+                                    D["scan_out"].ptr, n28, 4, init.ctypes.data)
+        lib.parallax_sort(K["sort_u32"], sort_u32[i].ptr, n26, 4)
+        lib.parallax_sort(K["sort_f64"], sort_f64[i].ptr, n25, 8)
+        kept_box[0] = lib.parallax_copy_if(K["flags"], K["scan_i32"], K["add_i32"], K["scatter"],
+                                           D["cif_in"].ptr, D["cif_out"].ptr, n28, 4, 0)
+
+    # ---- verification call (full size) against the oracle / exact closed forms
+    t0 = time.time()
+    step(nsteps - 1)
+    parity = verify(X, D, sort_u32[nsteps - 1], sort_f64[nsteps - 1], kept_box[0])
+    log(f"[bench] parity verified in {time.time() - t0:.1f}s: {parity}")
+
+    for i in range(args.warmup):
+        step(i)
+    lib.pxd_device_sync()
+    lib.pxr_prof_enable(1)
+    launches0 = stats()["launches"]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i)
+    lib.pxd_device_sync()
+    wall = time.perf_counter() - t0
+    assert stats()["launches"] - launches0 >= 7 * args.steps, "HIP path did not run"
+
+    kept = kept_box[0]
+    prims = [
+        ("reduce:f32", "reduce_f32", n28 * 4),
+        ("reduce:i64", "reduce_i64", n28 * 8),
+        ("scan:u32", "inclusive_scan_u32", 2 * n28 * 4),
+        ("exclusive_scan:u32", "exclusive_scan_u32", 2 * n28 * 4),
+        ("sort:u32", "sort_u32", 2 * n26 * 4),
+        ("sort:f64", "sort_f64", 2 * n25 * 8),
+        ("copy_if:i32", "copy_if_i32", n28 * 4 + kept * 4),
+    ]
+    per = {}
+    step_bytes = 0
+    for key, name, bytes_ in prims:
+        tot = ctypes.c_double()
+        calls = ctypes.c_uint64()
+        lib.pxr_prof_get(key.encode(), ctypes.byref(tot), ctypes.byref(calls))
+        ms = tot.value / max(calls.value, 1)
+        gbps = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        per[name] = {"ms": round(ms, 4), "alg_bytes": bytes_, "gbps": round(gbps, 1),
+                     "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 1)}
+        step_bytes += bytes_
+    lib.pxr_prof_enable(0)
+    ms_step = wall * 1e3 / args.steps
+    value = step_bytes / (ms_step * 1e-3) / 1e9
+    dom = max(per, key=lambda k: per[k]["ms"])
+    roof = {"bound": "hbm", "kernel": dom, "achieved": per[dom]["gbps"], "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(per[dom]["gbps"] / HBM_PEAK_GBPS, 4), "traffic": None,
+            "units_per_launch": "one call; algorithmic bytes as in per_primitive.alg_bytes"}
+    return {
+        "value": round(value, 1), "unit": "GB/s", "ms_per_step": round(ms_step, 3),
+        "dtype": "mixed(f32,i64,u32,f64,i32)",
+        "config": {"workload": "plain per-call primitives via the C ABI, HBM-resident operands: "
+                               "reduce 2^28 f32 + 2^28 i64, inclusive + exclusive scan 2^28 u32, "
+                               "sort 2^26 u32 + 2^25 f64, copy_if 2^28 i32 (x%2==0), seed 1",
+                   "scale": scale},
+        "roofline": roof, "per_primitive": per, "parity": parity,
+    }
+
+
+def verify(X, D, su, sf, kept):
+    """Full-size checks: integers, sort and compaction bit-exact; floats within the bound."""
+    import math
+    res = D["res"].download(np.uint8, 16)
+    out = {}
+    f32 = res[:4].view(np.float32)[0]
+    a = X["reduce_f32"]
+    exact = float(np.sum(a, dtype=np.float64))
+    ab = float(np.sum(np.abs(a), dtype=np.float64))
+    bound = 2 * math.ceil(math.log2(a.size)) * float(np.finfo(np.float32).eps) * ab
+    err = abs(float(f32) - exact)
+    assert err <= bound, ("reduce f32", err, bound)
+    out["reduce_f32_err_over_bound"] = err / bound
+    i64 = res[8:16].view(np.int64)[0]
+    assert i64 == np.sum(X["reduce_i64"], dtype=np.int64), "reduce i64"
+    s = X["scan_u32"]
+    cs = np.cumsum(s, dtype=np.uint32)
+    assert np.array_equal(D["scan_inplace"].download(np.uint32, s.size), cs), "inclusive scan"
+    ex = D["scan_out"].download(np.uint32, s.size)
+    assert ex[0] == 0 and np.array_equal(ex[1:], cs[:-1]), "exclusive scan"
+    assert np.array_equal(su.download(np.uint32, X["sort_u32"].size), np.sort(X["sort_u32"])), "sort u32"
+    assert sf.download(np.float64, X["sort_f64"].size).tobytes() == np.sort(X["sort_f64"]).tobytes(), "sort f64"
+    c = X["copy_if_i32"]
+    ref = c[np.fmod(c, 2) == 0]
+    assert kept == ref.size and np.array_equal(D["cif_out"].download(np.int32, kept), ref), "copy_if"
+    out["bit_exact"] = ["reduce_i64", "scans_u32", "sort_u32", "sort_f64", "copy_if_i32"]
+    return out
+
+
+# ------------------------------------------------------------------ CPU baseline
+CPU_SAMPLE = {"n_stream": 1 << 26, "n_sort_u32": 1 << 23, "n_sort_f64": 1 << 22}
+
+
+def cpu_baseline_child():
+    """Runs in a child process pinned to one core with PARALLAX_FORCE_CPU=1: the
+    reference's own fallback (oracle/_ref, built from its stdpar.hpp) on a bounded
+    sample of the same suite; prints one JSON line."""
+    import pxo
+    from parallax_amd import datagen
+
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    R = pxo.load_ref()
+    kind = "reference"
+    ns, nu, nf = CPU_SAMPLE["n_stream"], CPU_SAMPLE["n_sort_u32"], CPU_SAMPLE["n_sort_f64"]
+    f32 = datagen.normal_f32(1, ns, stream=1)
+    i64 = datagen.int_range(1, ns, -(1 << 20), 1 << 20, np.int64, stream=2)
+    u32 = datagen.int_range(1, ns, 0, 16, np.uint32, stream=3)
+    su = datagen.with_duplicates(datagen.full_u32(1, nu, stream=4), 1, 1, stream=40)
+    sf = datagen.with_duplicates(datagen.uniform01_f64(1, nf, stream=5), 1, 1, stream=50)
+    ci = datagen.full_i32(1, ns, stream=6)
+    out_u = np.empty_like(u32)
+    out_i = np.empty_like(ci)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+
+    def f(name, res, *a):
+        fn = getattr(R, name)
+        fn.restype = res
+        fn.argtypes = list(a)
+        return fn
+
+    if R is None:
+        raise SystemExit("oracle/_ref not built")
+    calls = [
+        ("reduce_f32", ns * 4, lambda: f("ref_reduce_f32", ctypes.c_float, vp, sz)(f32.ctypes.data, ns)),
+        ("reduce_i64", ns * 8, lambda: f("ref_reduce_i64", ctypes.c_int64, vp, sz)(i64.ctypes.data, ns)),
+        ("inclusive_scan_u32", 2 * ns * 4, lambda: f("ref_inclusive_scan_u32", None, vp, vp, sz)(u32.ctypes.data, out_u.ctypes.data, ns)),
+        ("exclusive_scan_u32", 2 * ns * 4, lambda: f("ref_exclusive_scan_u32", None, vp, vp, sz, ctypes.c_uint32)(u32.ctypes.data, out_u.ctypes.data, ns, 0)),
+        ("sort_u32", 2 * nu * 4, None),
+        ("sort_f64", 2 * nf * 8, None),
+        ("copy_if_i32", None, lambda: f("ref_copy_if_even_i32", sz, vp, vp, sz)(ci.ctypes.data, out_i.ctypes.data, ns)),
+    ]
+    times = {}
+    total_bytes = 0
+    for name, bytes_, fn in calls:
+        reps = []
+        for _ in range(3):
+            if name == "sort_u32":
+                w = su.copy()
+                t = time.perf_counter()
+                f("ref_sort_u32", None, vp, sz)(w.ctypes.data, nu)
+            elif name == "sort_f64":
+                w = sf.copy()
+                t = time.perf_counter()
+                f("ref_sort_f64", None, vp, sz)(w.ctypes.data, nf)
+            else:
+                t = time.perf_counter()
+                r = fn()
+            reps.append(time.perf_counter() - t)
+            if name == "copy_if_i32":
+                bytes_ = ns * 4 + int(r) * 4
+        times[name] = sorted(reps)[1]
+        total_bytes += bytes_
+    total = sum(times.values())
+    cpu = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    print(json.dumps({"value": round(total_bytes / total / 1e9, 3), "unit": "GB/s", "cores": 1,
+                      "kind": kind,
+                      "sample": (f"reference CPU fallback (stdpar.hpp host loops, g++ -O3, 1 core, median of 3) "
+                                 f"on the same suite at reduced sizes: streams 2^26, sort u32 2^23, "
+                                 f"sort f64 2^22; host {cpu}"),
+                      "seconds": {k: round(v, 4) for k, v in times.items()}}))
+
+
+def cpu_baseline():
+    env = dict(os.environ, PARALLAX_FORCE_CPU="1")
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--cpu-baseline-child"], env=env,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log("[bench] cpu baseline failed:", r.stderr[-2000:])
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+# ------------------------------------------------------------------ main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plain", choices=["plain", "batched", "served"])
+    ap.add_argument("--scale", type=float, default=1.0, help="size multiplier (1.0 = the configuration)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child()
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the primitives run on one device); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "plain":
+        r = plain_suite(args)
+    else:
+        print(f"bench.py: workload {args.workload} is not available yet", file=sys.stderr)
+        sys.exit(2)
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    line = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": r["dtype"], "data": "synthetic", "config": r["config"],
+            "roofline": r.get("roofline"), "cpu_baseline": cpu}
+    for k in ("per_primitive", "parity", "latency_ms", "req_per_s", "speedup_vs_plain"):
+        if k in r:
+            line[k] = r[k]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

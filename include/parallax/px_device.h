@@ -1,0 +1,136 @@
+/*
+ * parallax/px_device.h -- device-resident, stream-ordered entry points of the MI355X
+ * engine, and the batched (segmented) forms.
+ *
+ * NO COUNTERPART IN THE REFERENCE: the reference's primitives exist only as the
+ * synchronous, host-visible calls of parallax/runtime.h.  These entry points serve
+ * callers whose buffers already live in HBM (benchmarks, the request server, other
+ * frameworks): they take device pointers, enqueue on the context's HIP stream and
+ * return without waiting.  The synchronous C ABI of runtime.h runs on the same
+ * kernels.
+ *
+ * Every function returns 0 on success or a negative PX_E* code; pxr_last_error()
+ * describes the last failure of the calling thread.
+ */
+#ifndef PARALLAX_PX_DEVICE_H
+#define PARALLAX_PX_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "parallax/descriptor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PX_EOK 0
+#define PX_EARG (-1)
+#define PX_EDEVICE (-2)
+#define PX_ENODEV (-3)
+#define PX_ENOMEM (-4)
+#define PX_EUNSUPPORTED (-5)
+
+typedef struct pxd_ctx* pxd_ctx_t;
+
+/* ---- runtime information */
+int pxr_device_count(void);
+/* Fills name (NUL-terminated, <= name_len) and numbers; any pointer may be NULL. */
+int pxr_device_info(char* name, size_t name_len, int* compute_units, size_t* hbm_bytes,
+                    size_t* lds_per_block);
+int pxr_last_error(char* buf, size_t len);
+/* Counters of device work launched by this process: launches, reduce, scan, sort,
+ * compact, transform, segmented (7 values). */
+int pxr_stats(uint64_t* out7);
+const char* pxr_version(void);
+/* Arena of the C ABI: host mirror base, device base, capacity, bytes in use. */
+int pxr_arena_info(void** host_base, void** dev_base, size_t* capacity, size_t* used);
+/* The reference's relocation probe (tests/unit/test_physptr.cpp): the kernel receives
+ * the arena HOST pointer, relocates it (dev_base + (host_ptr - host_base)) and does
+ * r[i] = r[i]*2+1 for count floats. */
+int pxr_relocate_probe(void* arena_ptr, unsigned count);
+/* The descriptor (12 words) that a funnel key resolves to without registration, or
+ * PX_EUNSUPPORTED when the key names a functor funnel. */
+int pxr_builtin_descriptor(const char* key, unsigned* words12);
+/* Event timing of the C ABI's device work (per call, on its stream), keyed
+ * "<op>:<dtype>" with op in reduce, scan, exclusive_scan, sort, copy_if, partition and
+ * dtype in i32, u32, i64, u64, f32, f64.  enable(1) clears the table. */
+int pxr_prof_enable(int on);
+int pxr_prof_get(const char* key, double* total_ms, uint64_t* calls);
+
+/* ---- device memory helpers (for bindings without a GPU framework) */
+void* pxd_malloc(size_t bytes);
+void pxd_free(void* p);
+int pxd_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int pxd_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int pxd_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int pxd_memset(void* dst, int value, size_t bytes);
+int pxd_device_sync(void);
+
+/* ---- contexts: one HIP stream (hip_stream == NULL: the context creates its own) plus
+ * the scratch its kernels use.  Calls on one context run in order; different contexts
+ * may run concurrently. */
+pxd_ctx_t pxd_ctx_create(void* hip_stream);
+void pxd_ctx_destroy(pxd_ctx_t ctx);
+void* pxd_ctx_stream(pxd_ctx_t ctx);
+int pxd_sync(pxd_ctx_t ctx);
+/* Event timing on the context's stream: begin records a start event, end records a stop
+ * event, waits for it and returns the elapsed milliseconds in *ms. */
+int pxd_timer_begin(pxd_ctx_t ctx);
+int pxd_timer_end(pxd_ctx_t ctx, double* ms);
+
+/* ---- plain primitives (device pointers) */
+/* *d_out = sum of d_in[0..n) in the element type (one element written). */
+int pxd_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, void* d_out);
+int pxd_inclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n);
+/* h_init: host bytes of one element (NULL means 0). */
+int pxd_exclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n,
+                       const void* h_init);
+int pxd_sort(pxd_ctx_t ctx, int dtype, void* d_data, size_t n);
+/* Stable compaction of elements whose predicate (flags->fn, PX_PRED_UNIQUE for unique)
+ * holds; *d_count (device uint64) receives the kept count. */
+int pxd_copy_if(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void* d_out, size_t n,
+                uint64_t* d_count);
+/* Stable partition: kept first, the rest after; *d_num_true receives the split. */
+int pxd_partition(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void* d_out, size_t n,
+                  uint64_t* d_num_true);
+int pxd_transform(pxd_ctx_t ctx, const px_desc* map, const void* d_in, void* d_out, size_t n,
+                  const void* captures, size_t capture_size);
+int pxd_find(pxd_ctx_t ctx, const px_desc* pred, const void* d_in, size_t n, uint64_t* d_index);
+int pxd_argminmax(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, int want_max,
+                  int want_last, uint64_t* d_index);
+int pxd_mismatch(pxd_ctx_t ctx, int dtype, const void* d_a, const void* d_b, size_t n,
+                 uint64_t* d_index);
+/* Rewrite stored host addresses inside [host_base, host_base+bytes) to dev_base + offset. */
+int pxd_relocate_pointers(pxd_ctx_t ctx, uint64_t* d_ptrs, size_t n, uint64_t host_base,
+                          uint64_t dev_base, uint64_t bytes);
+
+/* ---- batched (segmented) forms: segment s is [h_offsets[s], h_offsets[s+1]) of the
+ * concatenated array (host offsets, nseg+1 entries).  One launch (sort: one fixed
+ * launch sequence) covers every segment; a segment's result is the same as the plain
+ * call on that segment alone. */
+int pxb_seg_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, const uint64_t* h_offsets,
+                   size_t nseg, void* d_out /* nseg elements */);
+int pxb_seg_inclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out,
+                           const uint64_t* h_offsets, size_t nseg);
+int pxb_seg_sort(pxd_ctx_t ctx, int dtype, void* d_data, const uint64_t* h_offsets, size_t nseg);
+/* Kept elements of segment s go to d_out + h_offsets[s]; d_counts[s] (device uint64)
+ * receives the kept count of segment s.  flags->fn == PX_PRED_UNIQUE gives unique. */
+int pxb_seg_copy_if(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void* d_out,
+                    const uint64_t* h_offsets, size_t nseg, uint64_t* d_counts);
+
+/* ---- graph plans: capture one batched call into a hipGraph once, replay it many
+ * times (same buffers, same offsets).  op: PX_OP_REDUCE, PX_OP_SCAN, PX_OP_SORT,
+ * PX_OP_FLAGS (segmented copy_if/unique with `flags`). */
+typedef struct pxb_plan* pxb_plan_t;
+pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flags,
+                           const void* d_in, void* d_out, const uint64_t* h_offsets, size_t nseg,
+                           uint64_t* d_counts);
+int pxb_plan_launch(pxb_plan_t plan);
+void pxb_plan_destroy(pxb_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARALLAX_PX_DEVICE_H */

@@ -1,0 +1,349 @@
+// dev_common.hpp -- device-side building blocks shared by every gfx950 primitive kernel.
+//
+//  * element traits: per element type T, the carry type A the kernels accumulate in
+//      int32/uint32 -> uint32 (wrapping), int64/uint64 -> uint64 (wrapping),
+//      float -> double, double -> double-double (hi, lo).
+//    Integer sums are exact modulo 2^w in any order; the float carries keep every
+//    reduce / scan result within a few units of the last place of the exact sum.
+//  * wave64 cross-lane steps on DPP (row_shr, row_bcast15/31, quad_perm, row mirrors,
+//    wave_shr) -- no LDS inside a wave; LDS is used only between the waves of a block.
+//  * agent-scope status words for single-pass decoupled look-back: each 32-bit word of
+//    a carry travels in a 64-bit atomic word {flag:32 | payload:32}, so a reader never
+//    needs a fence and never sees a torn value (all words must carry the same flag).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace pxk {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- double-double
+struct dd {
+  double hi, lo;
+};
+
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+  // TwoSum of the high parts, then fold both low parts and renormalise.
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b.hi - bb);
+  e += a.lo + b.lo;
+  const double h = s + e;
+  const double l = e - (h - s);
+  return dd{h, l};
+}
+
+// ---------------------------------------------------------------- carry traits
+template <class T>
+struct Traits;
+template <>
+struct Traits<int32_t> {
+  using A = uint32_t;
+  static constexpr bool is_float = false;
+};
+template <>
+struct Traits<uint32_t> {
+  using A = uint32_t;
+  static constexpr bool is_float = false;
+};
+template <>
+struct Traits<int64_t> {
+  using A = uint64_t;
+  static constexpr bool is_float = false;
+};
+template <>
+struct Traits<uint64_t> {
+  using A = uint64_t;
+  static constexpr bool is_float = false;
+};
+template <>
+struct Traits<float> {
+  using A = double;
+  static constexpr bool is_float = true;
+};
+template <>
+struct Traits<double> {
+  using A = dd;
+  static constexpr bool is_float = true;
+};
+
+__device__ __forceinline__ uint32_t acc_add(uint32_t a, uint32_t b) { return a + b; }
+__device__ __forceinline__ uint64_t acc_add(uint64_t a, uint64_t b) { return a + b; }
+__device__ __forceinline__ double acc_add(double a, double b) { return a + b; }
+__device__ __forceinline__ dd acc_add(dd a, dd b) { return dd_add(a, b); }
+
+template <class A>
+__device__ __forceinline__ A acc_zero() {
+  return A{};
+}
+template <>
+__device__ __forceinline__ dd acc_zero<dd>() {
+  return dd{0.0, 0.0};
+}
+
+__device__ __forceinline__ uint32_t to_acc(int32_t x) { return static_cast<uint32_t>(x); }
+__device__ __forceinline__ uint32_t to_acc(uint32_t x) { return x; }
+__device__ __forceinline__ uint64_t to_acc(int64_t x) { return static_cast<uint64_t>(x); }
+__device__ __forceinline__ uint64_t to_acc(uint64_t x) { return x; }
+__device__ __forceinline__ double to_acc(float x) { return static_cast<double>(x); }
+__device__ __forceinline__ dd to_acc(double x) { return dd{x, 0.0}; }
+
+template <class T>
+__device__ __forceinline__ T from_acc(typename Traits<T>::A a);
+template <>
+__device__ __forceinline__ int32_t from_acc<int32_t>(uint32_t a) {
+  return static_cast<int32_t>(a);
+}
+template <>
+__device__ __forceinline__ uint32_t from_acc<uint32_t>(uint32_t a) {
+  return a;
+}
+template <>
+__device__ __forceinline__ int64_t from_acc<int64_t>(uint64_t a) {
+  return static_cast<int64_t>(a);
+}
+template <>
+__device__ __forceinline__ uint64_t from_acc<uint64_t>(uint64_t a) {
+  return a;
+}
+template <>
+__device__ __forceinline__ float from_acc<float>(double a) {
+  return static_cast<float>(a);  // round to nearest even
+}
+template <>
+__device__ __forceinline__ double from_acc<double>(dd a) {
+  return a.hi + a.lo;
+}
+
+// ---------------------------------------------------------------- DPP moves
+// Move a value of any 4-byte-multiple type across lanes with one DPP mov per dword.
+// Lanes whose source is outside the row / disabled by the masks receive 0.
+template <int CTRL, int ROW_MASK, int BANK_MASK, class V>
+__device__ __forceinline__ V dpp_mov(V v) {
+  static_assert(sizeof(V) % 4 == 0, "dpp_mov needs dword-sized values");
+  constexpr int W = sizeof(V) / 4;
+  union U {
+    V v;
+    int w[W];
+  } in, out;
+  in.v = v;
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+    out.w[i] = __builtin_amdgcn_update_dpp(0, in.w[i], CTRL, ROW_MASK, BANK_MASK, false);
+  return out.v;
+}
+
+template <class V>
+__device__ __forceinline__ V read_lane(V v, int lane) {
+  constexpr int W = sizeof(V) / 4;
+  union U {
+    V v;
+    int w[W];
+  } in, out;
+  in.v = v;
+#pragma unroll
+  for (int i = 0; i < W; ++i) out.w[i] = __builtin_amdgcn_readlane(in.w[i], lane);
+  return out.v;
+}
+
+// DPP control codes (GFX9 encoding).
+constexpr int kQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kRowShr1 = 0x111;
+constexpr int kRowShr2 = 0x112;
+constexpr int kRowShr4 = 0x114;
+constexpr int kRowShr8 = 0x118;
+constexpr int kWaveShr1 = 0x138;
+constexpr int kRowMirror = 0x140;
+constexpr int kRowHalfMirror = 0x141;
+constexpr int kRowBcast15 = 0x142;
+constexpr int kRowBcast31 = 0x143;
+
+// Inclusive scan across the 64 lanes of a wave: Hillis-Steele inside each 16-lane row
+// (row_shr 1,2,4,8), then row_bcast15 (rows 1,3) and row_bcast31 (rows 2,3).
+template <class A>
+__device__ __forceinline__ A wave_incl_scan(A v) {
+  v = acc_add(v, dpp_mov<kRowShr1, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowShr2, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowShr4, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowShr8, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowBcast15, 0xa, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowBcast31, 0xc, 0xf>(v));
+  return v;
+}
+
+// Exclusive counterpart of an inclusive wave scan: shift right by one lane (lane 0 -> 0).
+template <class A>
+__device__ __forceinline__ A wave_shift_right1(A v) {
+  return dpp_mov<kWaveShr1, 0xf, 0xf>(v);
+}
+
+// Sum over the 64 lanes, identical in every lane: butterfly inside each row
+// (quad xor 1, quad xor 2, half-row mirror, row mirror), then the four row totals in
+// a fixed order through readlane.
+template <class A>
+__device__ __forceinline__ A wave_sum(A v) {
+  v = acc_add(v, dpp_mov<kQuadXor1, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kQuadXor2, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowHalfMirror, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowMirror, 0xf, 0xf>(v));
+  A r0 = read_lane(v, 0), r1 = read_lane(v, 16), r2 = read_lane(v, 32), r3 = read_lane(v, 48);
+  return acc_add(acc_add(r0, r1), acc_add(r2, r3));
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// ---------------------------------------------------------------- agent-scope words
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent32(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Store / load a carry value word by word through agent-scope atomics (used for the
+// per-block partials that another block combines).
+template <class A>
+__device__ __forceinline__ void st_carry(A* p, A v) {
+  constexpr int W = sizeof(A) / 4;
+  union U {
+    A v;
+    uint32_t w[W];
+  } u;
+  u.v = v;
+  uint32_t* q = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+  for (int i = 0; i < W; ++i) st_agent32(q + i, u.w[i]);
+}
+template <class A>
+__device__ __forceinline__ A ld_carry(const A* p) {
+  constexpr int W = sizeof(A) / 4;
+  union U {
+    A v;
+    uint32_t w[W];
+  } u;
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (int i = 0; i < W; ++i) u.w[i] = ld_agent32(q + i);
+  return u.v;
+}
+
+__device__ __forceinline__ uint32_t next_epoch(uint32_t e) { return e >= kEpochMax ? 1u : e + 1u; }
+
+// Block start: take a ticket and read the epoch (thread 0), broadcast through LDS.
+__device__ __forceinline__ void block_begin(Ctl* ctl, uint32_t* s_ticket, uint32_t* s_epoch) {
+  if (threadIdx.x == 0) {
+    *s_ticket = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *s_epoch = ld_agent32(&ctl->epoch);
+  }
+  __syncthreads();
+}
+
+// Block end (thread 0 only): the last block of the launch resets ticket/done and
+// advances the epoch for the next launch on this context.
+__device__ __forceinline__ void block_end(Ctl* ctl, uint32_t epoch) {
+  uint32_t d = __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (d == gridDim.x - 1) {
+    st_agent32(&ctl->ticket, 0u);
+    st_agent32(&ctl->done, 0u);
+    st_agent32(&ctl->epoch, next_epoch(epoch));
+  }
+}
+
+// ---------------------------------------------------------------- look-back status
+// Tile status: W = sizeof(A)/4 words per tile, word = (flag << 32) | payload, with
+// flag = (epoch << 2) | state, state 1 = aggregate, 2 = inclusive prefix.
+constexpr uint32_t kStAgg = 1u;
+constexpr uint32_t kStIncl = 2u;
+
+template <class A>
+__device__ __forceinline__ void publish(uint64_t* status, uint64_t tile, uint32_t epoch,
+                                        uint32_t state, A v) {
+  constexpr int W = sizeof(A) / 4;
+  union U {
+    A v;
+    uint32_t w[W];
+  } u;
+  u.v = v;
+  const uint64_t flag = static_cast<uint64_t>((epoch << 2) | state) << 32;
+#pragma unroll
+  for (int i = 0; i < W; ++i) st_agent(status + tile * W + i, flag | u.w[i]);
+}
+
+// Read tile `tile`'s status.  Returns the state (0 = not yet valid for this epoch).
+template <class A>
+__device__ __forceinline__ uint32_t peek(const uint64_t* status, uint64_t tile, uint32_t epoch,
+                                         A& v) {
+  constexpr int W = sizeof(A) / 4;
+  union U {
+    A v;
+    uint32_t w[W];
+  } u;
+  uint32_t f0 = 0;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const uint64_t word = ld_agent(status + tile * W + i);
+    const uint32_t f = static_cast<uint32_t>(word >> 32);
+    if (i == 0) f0 = f;
+    ok = ok && (f == f0);
+    u.w[i] = static_cast<uint32_t>(word);
+  }
+  v = u.v;
+  if (!ok || (f0 >> 2) != epoch) return 0u;
+  return f0 & 3u;
+}
+
+// Decoupled look-back (Merrill & Garland 2016), run by one whole wave for tile `tile`
+// (> first).  Lane L inspects tile (pred - L); tiles before `first` (the first tile of
+// the segment) count as an inclusive prefix of 0.  The wave waits until every lane's
+// tile has published, then sums the aggregates up to and including the closest
+// inclusive prefix.  Returns the exclusive prefix of `tile` in every lane.  Every
+// predecessor has already taken its ticket (tickets are handed out in order), so every
+// wait ends.
+template <class A>
+__device__ __forceinline__ A lookback(const uint64_t* status, uint64_t tile, int64_t first,
+                                      uint32_t epoch) {
+  const int lane = lane_id();
+  A run = acc_zero<A>();
+  int64_t pred = static_cast<int64_t>(tile) - 1;
+  while (true) {
+    const int64_t t = pred - lane;
+    A v = acc_zero<A>();
+    uint32_t st = (t >= first) ? peek<A>(status, static_cast<uint64_t>(t), epoch, v) : kStIncl;
+    while (__ballot(st == 0u) != 0ull) {
+      if (st == 0u) {
+        __builtin_amdgcn_s_sleep(1);
+        st = peek<A>(status, static_cast<uint64_t>(t), epoch, v);
+      }
+    }
+    const uint64_t incl = __ballot(st == kStIncl);
+    if (incl != 0ull) {
+      const int first = __ffsll(static_cast<unsigned long long>(incl)) - 1;
+      A c = (lane <= first) ? v : acc_zero<A>();
+      run = acc_add(run, wave_sum(c));
+      break;
+    }
+    run = acc_add(run, wave_sum(v));
+    pred -= kWave;
+  }
+  return run;
+}
+
+// ---------------------------------------------------------------- vector IO
+template <class T, int N>
+struct alignas(sizeof(T) * N) Vec {
+  T v[N];
+};
+
+}  // namespace pxk

@@ -1,0 +1,69 @@
+// dev_pred.hpp -- predicate semantics on the device (descriptor predicates of
+// parallax/descriptor.h).  Integer remainder follows C++ truncation; x % 0 is false and
+// x % -1 is 0 (both undefined in C++; pinned here).  Floats use IEEE comparisons, so a
+// NaN fails every ordered predicate; MOD_EQ on floats uses fmod.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+#include "parallax/descriptor.h"
+
+namespace pxk {
+
+template <class T>
+__device__ __forceinline__ T bits_as(uint64_t b) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, static_cast<uint32_t>(b));
+  } else {
+    return __builtin_bit_cast(T, b);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ bool pred_eval(uint32_t fn, uint32_t flags, T c0, T c1, T x) {
+  bool r = false;
+  switch (fn) {
+    case PX_PRED_TRUE: r = true; break;
+    case PX_PRED_EQ: r = (x == c0); break;
+    case PX_PRED_NE: r = (x != c0); break;
+    case PX_PRED_LT: r = (x < c0); break;
+    case PX_PRED_LE: r = (x <= c0); break;
+    case PX_PRED_GT: r = (x > c0); break;
+    case PX_PRED_GE: r = (x >= c0); break;
+    case PX_PRED_IN_RANGE: r = (c0 <= x) && (x < c1); break;
+    case PX_PRED_MOD_EQ:
+      if constexpr (std::is_floating_point_v<T>) {
+        r = (fmod(x, c0) == c1);
+      } else {
+        if (c0 == T(0)) {
+          r = false;
+        } else if constexpr (std::is_signed_v<T>) {
+          const T rem = (c0 == T(-1)) ? T(0) : static_cast<T>(x % c0);
+          r = (rem == c1);
+        } else {
+          r = (static_cast<T>(x % c0) == c1);
+        }
+      }
+      break;
+    case PX_PRED_NONZERO: r = (x != T(0)); break;
+    case PX_PRED_BITS_ANY:
+      if constexpr (std::is_integral_v<T>) r = (x & c0) != 0;
+      break;
+    default: r = false; break;
+  }
+  return (flags & PX_FLAG_NEGATE) ? !r : r;
+}
+
+// Predicate with its constants unpacked once per kernel.
+template <class T>
+struct Pred {
+  uint32_t fn, flags;
+  T c0, c1;
+  __device__ __forceinline__ Pred(const px_desc& d)
+      : fn(d.fn), flags(d.flags), c0(bits_as<T>(d.c0)), c1(bits_as<T>(d.c1)) {}
+  __device__ __forceinline__ bool operator()(T x) const { return pred_eval<T>(fn, flags, c0, c1, x); }
+};
+
+}  // namespace pxk

@@ -1,0 +1,426 @@
+// engine.cpp -- stream contexts, tile tables and the primitive runners.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+#include "dev_tiles.hpp"
+
+namespace parallax {
+
+namespace {
+thread_local int t_err_code = 0;
+thread_local std::string t_err_text;
+EngineStats g_stats;
+}  // namespace
+
+void set_last_error(int code, const std::string& what) {
+  t_err_code = code;
+  t_err_text = what;
+}
+int last_error_code() { return t_err_code; }
+const char* last_error_text() { return t_err_text.c_str(); }
+EngineStats& stats() { return g_stats; }
+
+HipBackend* global_backend() {
+  static std::mutex mu;
+  static std::unique_ptr<HipBackend> be;
+  static bool tried = false;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!tried) {
+    tried = true;
+    auto b = std::make_unique<HipBackend>();
+    if (b->initialize()) be = std::move(b);
+  }
+  return be.get();
+}
+
+// ------------------------------------------------------------------ StreamContext
+StreamContext::StreamContext(HipBackend* be, hipStream_t s) : be_(be) {
+  if (!be_ || !be_->valid()) return;
+  if (hipSetDevice(be_->device()) != hipSuccess) return;
+  if (s) {
+    stream_ = s;
+  } else {
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return;
+    own_stream_ = true;
+  }
+  cus_ = std::max(1, be_->capabilities().compute_units);
+  if (hipMalloc(reinterpret_cast<void**>(&ctl_), sizeof(pxk::Ctl)) != hipSuccess) return;
+  pxk::Ctl init;
+  std::memset(&init, 0, sizeof(init));
+  init.epoch = 1;
+  if (hipMemcpy(ctl_, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return;
+  if (hipEventCreateWithFlags(&pin_ev_, hipEventDisableTiming) != hipSuccess) return;
+  ok_ = true;
+  if (!reserve(1u << 16, 64u << 10, 1024)) ok_ = false;
+}
+
+StreamContext::~StreamContext() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (ctl_) (void)hipFree(ctl_);
+  if (status_) (void)hipFree(status_);
+  if (partials_) (void)hipFree(partials_);
+  if (segcnt_) (void)hipFree(segcnt_);
+  if (work_) (void)hipFree(work_);
+  if (pin_) (void)hipHostFree(pin_);
+  if (pin_ev_) (void)hipEventDestroy(pin_ev_);
+  if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+bool StreamContext::grow(void** p, size_t* cap, size_t need, bool zero) {
+  if (need <= *cap) return true;
+  if (frozen_) return false;  // no reallocation inside a graph capture
+  size_t ncap = std::max<size_t>(need, *cap * 2);
+  ncap = (ncap + 4095) & ~size_t(4095);
+  if (hipStreamSynchronize(stream_) != hipSuccess) return false;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, ncap) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return false;
+  }
+  if (zero && hipMemset(*p, 0, ncap) != hipSuccess) return false;
+  *cap = ncap;
+  return true;
+}
+
+bool StreamContext::reserve(size_t status_words, size_t partial_bytes, size_t seg_counters) {
+  if (!grow(reinterpret_cast<void**>(&status_), &status_cap_, status_words * 8, true)) return false;
+  if (!grow(&partials_, &partial_cap_, partial_bytes, false)) return false;
+  if (!grow(reinterpret_cast<void**>(&segcnt_), &segcnt_cap_, seg_counters * 4, true)) return false;
+  return true;
+}
+
+pxk::Scratch StreamContext::scratch() const {
+  pxk::Scratch s;
+  s.ctl = ctl_;
+  s.status = status_;
+  s.status_words = status_cap_ / 8;
+  s.partials = partials_;
+  s.partial_bytes = partial_cap_;
+  s.seg_cnt = segcnt_;
+  s.seg_counters = segcnt_cap_ / 4;
+  s.max_blocks = static_cast<uint32_t>(cus_) * 8u;
+  return s;
+}
+
+bool StreamContext::upload(const void* host, size_t bytes, void** dev_out) {
+  if (frozen_) {
+    // replaying a warmed-up call: the tables are already on the device
+    if (bytes != last_upload_ || !work_) return false;
+    *dev_out = work_;
+    return true;
+  }
+  if (!grow(&work_, &work_cap_, bytes, false)) return false;
+  last_upload_ = bytes;
+  if (pin_pending_) {
+    if (hipEventSynchronize(pin_ev_) != hipSuccess) return false;
+    pin_pending_ = false;
+  }
+  if (bytes > pin_cap_) {
+    if (pin_) (void)hipHostFree(pin_);
+    pin_ = nullptr;
+    pin_cap_ = 0;
+    const size_t ncap = (std::max(bytes, size_t(1) << 16) + 4095) & ~size_t(4095);
+    if (hipHostMalloc(&pin_, ncap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      pin_ = nullptr;
+      return false;
+    }
+    pin_cap_ = ncap;
+  }
+  std::memcpy(pin_, host, bytes);
+  if (hipMemcpyAsync(work_, pin_, bytes, hipMemcpyHostToDevice, stream_) != hipSuccess) return false;
+  if (hipEventRecord(pin_ev_, stream_) != hipSuccess) return false;
+  pin_pending_ = true;
+  *dev_out = work_;
+  return true;
+}
+
+bool StreamContext::note_status_launch() {
+  if (++status_launches_ < pxk::kEpochMax - 2 || frozen_) return true;
+  // About to wrap the 30-bit epoch: clear every status word and restart at epoch 1.
+  if (hipStreamSynchronize(stream_) != hipSuccess) return false;
+  if (status_ && hipMemset(status_, 0, status_cap_) != hipSuccess) return false;
+  pxk::Ctl init;
+  std::memset(&init, 0, sizeof(init));
+  init.epoch = 1;
+  if (hipMemcpy(ctl_, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return false;
+  status_launches_ = 1;
+  return true;
+}
+
+// ------------------------------------------------------------------ tile tables
+void build_tile_tables(const SegSpec& sp, uint32_t tile, uint32_t vec, int64_t shift,
+                       TileTables& t) {
+  t.tile_off.resize(sp.nseg + 1);
+  t.tile_seg.clear();
+  uint32_t acc = 0;
+  for (uint32_t s = 0; s < sp.nseg; ++s) {
+    t.tile_off[s] = acc;
+    const int64_t lo = shift + static_cast<int64_t>(sp.lo[s]);
+    const int64_t hi = shift + static_cast<int64_t>(sp.hi[s]);
+    const int64_t w = lo & ~static_cast<int64_t>(vec - 1);
+    uint64_t nt = hi > w ? static_cast<uint64_t>((hi - w + tile - 1) / tile) : 1;
+    if (nt == 0) nt = 1;
+    for (uint64_t k = 0; k < nt; ++k) t.tile_seg.push_back(s);
+    acc += static_cast<uint32_t>(nt);
+  }
+  t.tile_off[sp.nseg] = acc;
+  t.ntiles = acc;
+}
+
+namespace {
+
+bool valid_dtype(int d) { return d >= 0 && d < PX_NUM_DTYPES; }
+
+int dev_err(hipError_t e, const char* what) {
+  if (e == hipSuccess) return PX_OK;
+  set_last_error(PX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return PX_ERR_DEVICE;
+}
+
+// Zero `bytes` (multiple of 4) of device memory with a kernel (graph-safe).
+int zero_dev(StreamContext& c, void* p, size_t bytes) {
+  return dev_err(pxk::fill_u32(static_cast<uint32_t*>(p), bytes / 4, 0u, c.stream()), "fill");
+}
+
+pxk::SegGeom plain_geom(uint64_t n, int dtype, int k, const void* in) {
+  pxk::SegGeom g{};
+  const uint32_t tile = pxk::tile_elems(dtype, k);
+  const int64_t shift = pxk::elem_shift(in, pxk::dtype_size(dtype));
+  uint64_t nt = (static_cast<uint64_t>(shift) + n + tile - 1) / tile;
+  if (nt == 0) nt = 1;
+  g.n = n;
+  g.ntiles = static_cast<uint32_t>(nt);
+  g.nseg = 1;
+  return g;
+}
+
+bool valid_spec(const SegSpec& sp) {
+  if (!sp.lo || !sp.hi) return false;
+  for (uint32_t s = 0; s < sp.nseg; ++s)
+    if (sp.hi[s] < sp.lo[s]) return false;
+  return true;
+}
+
+// Upload segment bounds + tile tables as one blob and fill a segmented geometry.
+int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, int k, const void* in,
+             pxk::SegGeom& g, TileTables& tt) {
+  const uint32_t nseg = sp.nseg;
+  const uint32_t tile = pxk::tile_elems(dtype, k);
+  const uint32_t vec = static_cast<uint32_t>(16 / pxk::dtype_size(dtype));
+  const int64_t shift = pxk::elem_shift(in, pxk::dtype_size(dtype));
+  build_tile_tables(sp, tile, vec, shift, tt);
+  const size_t b64 = static_cast<size_t>(nseg) * sizeof(uint64_t);
+  const size_t o_hi = b64, o_out = 2 * b64;
+  const size_t o_toff = (sp.out ? 3 : 2) * b64;
+  const size_t b_toff = (nseg + 1) * sizeof(uint32_t);
+  const size_t o_tseg = (o_toff + b_toff + 15) & ~size_t(15);
+  const size_t b_tseg = static_cast<size_t>(tt.ntiles) * sizeof(uint32_t);
+  std::vector<unsigned char> blob(o_tseg + b_tseg);
+  std::memcpy(blob.data(), sp.lo, b64);
+  std::memcpy(blob.data() + o_hi, sp.hi, b64);
+  if (sp.out) std::memcpy(blob.data() + o_out, sp.out, b64);
+  std::memcpy(blob.data() + o_toff, tt.tile_off.data(), b_toff);
+  if (b_tseg) std::memcpy(blob.data() + o_tseg, tt.tile_seg.data(), b_tseg);
+  void* dev = nullptr;
+  if (!c.upload(blob.data(), blob.size(), &dev)) {
+    set_last_error(PX_ERR_NOMEM, "table upload");
+    return PX_ERR_NOMEM;
+  }
+  unsigned char* d = static_cast<unsigned char*>(dev);
+  g.seg_lo = reinterpret_cast<const uint64_t*>(d);
+  g.seg_hi = reinterpret_cast<const uint64_t*>(d + o_hi);
+  g.seg_out = sp.out ? reinterpret_cast<const uint64_t*>(d + o_out) : nullptr;
+  g.tile_off = reinterpret_cast<const uint32_t*>(d + o_toff);
+  g.tile_seg = reinterpret_cast<const uint32_t*>(d + o_tseg);
+  g.n = 0;
+  g.ntiles = tt.ntiles;
+  g.nseg = nseg;
+  return PX_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ runners
+int run_reduce(StreamContext& c, int dtype, const void* in, uint64_t n, void* out_dev) {
+  if (!valid_dtype(dtype) || !out_dev || (n && !in)) return PX_ERR_ARG;
+  ++g_stats.reduce;
+  ++g_stats.launches;
+  if (n == 0) return zero_dev(c, out_dev, pxk::dtype_size(dtype));
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
+  return dev_err(pxk::reduce(dtype, in, n, out_dev, c.scratch(), c.stream()), "reduce");
+}
+
+int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n, bool exclusive,
+             const void* init_host) {
+  if (!valid_dtype(dtype) || (n && (!in || !out))) return PX_ERR_ARG;
+  if (n == 0) return PX_OK;
+  ++g_stats.scan;
+  ++g_stats.launches;
+  const pxk::SegGeom g = plain_geom(n, dtype, pxk::kScanK, in);
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0, 0))
+    return PX_ERR_NOMEM;
+  if (!c.note_status_launch()) return PX_ERR_DEVICE;
+  return dev_err(pxk::scan(dtype, in, out, g, exclusive, init_host, c.scratch(), c.stream()), "scan");
+}
+
+int run_sort(StreamContext& c, int dtype, void* data, uint64_t n) {
+  if (!valid_dtype(dtype) || (n && !data)) return PX_ERR_ARG;
+  if (n <= 1) return PX_OK;
+  ++g_stats.sort;
+  ++g_stats.launches;
+  return dev_err(pxk::sort(dtype, data, n, c.stream()), "sort");
+}
+
+int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
+                uint64_t* count_dev) {
+  if (!valid_dtype(flags.dtype) || !count_dev || (n && (!in || !out))) return PX_ERR_ARG;
+  ++g_stats.compact;
+  ++g_stats.launches;
+  if (n == 0) return zero_dev(c, count_dev, sizeof(uint64_t));
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::kScanK, in);
+  const size_t words = n >= (1ull << 32) ? 2 : 1;
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0, 0)) return PX_ERR_NOMEM;
+  if (!c.note_status_launch()) return PX_ERR_DEVICE;
+  return dev_err(pxk::compact(flags, PX_SCATTER_COMPACT, in, out, g, count_dev, nullptr,
+                              c.scratch(), c.stream()),
+                 "compact");
+}
+
+int run_partition(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
+                  uint64_t* num_true_dev) {
+  if (!valid_dtype(flags.dtype) || !num_true_dev || (n && (!in || !out))) return PX_ERR_ARG;
+  ++g_stats.compact;
+  ++g_stats.launches;
+  if (n == 0) return zero_dev(c, num_true_dev, sizeof(uint64_t));
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
+  int rc = dev_err(pxk::count_if(flags, in, n, num_true_dev, c.scratch(), c.stream()), "count");
+  if (rc) return rc;
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::kScanK, in);
+  const size_t words = n >= (1ull << 32) ? 2 : 1;
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0, 0)) return PX_ERR_NOMEM;
+  if (!c.note_status_launch()) return PX_ERR_DEVICE;
+  return dev_err(pxk::compact(flags, PX_SCATTER_PARTITION, in, out, g, nullptr, num_true_dev,
+                              c.scratch(), c.stream()),
+                 "partition");
+}
+
+int run_transform(StreamContext& c, const px_desc& d, const void* in, void* out, uint64_t n,
+                  const void* captures, size_t capture_size, uint64_t host_base,
+                  uint64_t dev_base, uint64_t arena_bytes) {
+  if (!valid_dtype(d.dtype) || !valid_dtype(d.aux) || d.fn >= PX_NUM_MAPOPS) return PX_ERR_ARG;
+  if (n && (!in || !out)) return PX_ERR_ARG;
+  ++g_stats.transform;
+  ++g_stats.launches;
+  pxk::MapArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.count_lo = static_cast<uint32_t>(n);
+  a.host_base = host_base;
+  a.dev_base = dev_base;
+  a.arena_bytes = arena_bytes;
+  const size_t cs = std::min(capture_size, sizeof(a.cap));
+  if (captures && cs) std::memcpy(a.cap, captures, cs);
+  a.cap_size = captures ? static_cast<uint32_t>(cs) : 0u;
+  return dev_err(pxk::transform(d, in, out, n, a, c.stream()), "transform");
+}
+
+int run_find(StreamContext& c, const px_desc& pred, const void* in, uint64_t n, uint64_t* out_dev) {
+  if (!valid_dtype(pred.dtype) || !out_dev) return PX_ERR_ARG;
+  ++g_stats.launches;
+  return dev_err(pxk::find_first(pred, in, n, out_dev, c.scratch(), c.stream()), "find");
+}
+
+int run_argminmax(StreamContext& c, int dtype, const void* in, uint64_t n, bool want_max,
+                  bool want_last, uint64_t* out_dev) {
+  if (!valid_dtype(dtype) || !out_dev) return PX_ERR_ARG;
+  ++g_stats.launches;
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
+  return dev_err(pxk::argminmax(dtype, in, n, want_max, want_last, out_dev, c.scratch(), c.stream()),
+                 "argminmax");
+}
+
+int run_mismatch(StreamContext& c, int dtype, const void* a, const void* b, uint64_t n,
+                 uint64_t* out_dev) {
+  if (!valid_dtype(dtype) || !out_dev) return PX_ERR_ARG;
+  ++g_stats.launches;
+  return dev_err(pxk::mismatch(dtype, a, b, n, out_dev, c.scratch(), c.stream()), "mismatch");
+}
+
+int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& sp, void* out_dev) {
+  if (!valid_dtype(dtype) || !out_dev || !in || !valid_spec(sp)) return PX_ERR_ARG;
+  if (sp.nseg == 0) return PX_OK;
+  ++g_stats.seg;
+  ++g_stats.launches;
+  pxk::SegGeom g;
+  TileTables tt;
+  int rc = seg_geom(c, sp, dtype, pxk::kSegReduceK, in, g, tt);
+  if (rc) return rc;
+  if (!c.reserve(0, static_cast<size_t>(g.ntiles) * 16, sp.nseg)) return PX_ERR_NOMEM;
+  return dev_err(pxk::seg_reduce(dtype, in, g, out_dev, c.scratch(), c.stream()), "seg_reduce");
+}
+
+int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const SegSpec& sp) {
+  if (!valid_dtype(dtype) || !in || !out || !valid_spec(sp)) return PX_ERR_ARG;
+  if (sp.nseg == 0) return PX_OK;
+  ++g_stats.seg;
+  ++g_stats.launches;
+  pxk::SegGeom g;
+  TileTables tt;
+  int rc = seg_geom(c, sp, dtype, pxk::kScanK, in, g, tt);
+  if (rc) return rc;
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0, 0))
+    return PX_ERR_NOMEM;
+  if (!c.note_status_launch()) return PX_ERR_DEVICE;
+  return dev_err(pxk::scan(dtype, in, out, g, false, nullptr, c.scratch(), c.stream()), "seg_scan");
+}
+
+int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void* out,
+                    const SegSpec& sp, uint64_t* counts_dev) {
+  if (!valid_dtype(flags.dtype) || !in || !out || !counts_dev || !valid_spec(sp)) return PX_ERR_ARG;
+  if (sp.nseg == 0) return PX_OK;
+  ++g_stats.seg;
+  ++g_stats.launches;
+  pxk::SegGeom g;
+  TileTables tt;
+  int rc = seg_geom(c, sp, flags.dtype, pxk::kScanK, in, g, tt);
+  if (rc) return rc;
+  if (!c.reserve(static_cast<size_t>(g.ntiles), 0, 0)) return PX_ERR_NOMEM;
+  if (!c.note_status_launch()) return PX_ERR_DEVICE;
+  return dev_err(pxk::compact(flags, PX_SCATTER_COMPACT, in, out, g, counts_dev, nullptr,
+                              c.scratch(), c.stream()),
+                 "seg_compact");
+}
+
+int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
+  if (!valid_dtype(dtype) || !data || !valid_spec(sp)) return PX_ERR_ARG;
+  if (sp.nseg == 0) return PX_OK;
+  ++g_stats.seg;
+  ++g_stats.launches;
+  pxk::SegSortPlan p;
+  if (!pxk::seg_sort_plan(dtype, sp.lo, sp.hi, sp.nseg, p)) return PX_ERR_ARG;
+  // blob: interleaved bounds (lo, hi) as u64, then the block tables (u32)
+  const size_t b_b = static_cast<size_t>(sp.nseg) * 2 * sizeof(uint64_t);
+  std::vector<unsigned char> blob(b_b + p.tables.size() * sizeof(uint32_t));
+  uint64_t* bounds = reinterpret_cast<uint64_t*>(blob.data());
+  for (uint32_t s = 0; s < sp.nseg; ++s) {
+    bounds[2 * s] = sp.lo[s];
+    bounds[2 * s + 1] = sp.hi[s];
+  }
+  if (!p.tables.empty())
+    std::memcpy(blob.data() + b_b, p.tables.data(), p.tables.size() * sizeof(uint32_t));
+  void* dev = nullptr;
+  if (!c.upload(blob.data(), blob.size(), &dev)) return PX_ERR_NOMEM;
+  const uint64_t* bd = static_cast<const uint64_t*>(dev);
+  const uint32_t* tab_d = reinterpret_cast<const uint32_t*>(static_cast<unsigned char*>(dev) + b_b);
+  return dev_err(pxk::seg_sort_run(p, data, bd, tab_d, c.stream()), "seg_sort");
+}
+
+}  // namespace parallax

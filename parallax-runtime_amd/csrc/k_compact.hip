@@ -1,0 +1,278 @@
+// k_compact.hip -- stable stream compaction for gfx950: copy_if, remove_if (negated
+// predicate), unique (adjacent-difference flags) and partition, in one pass.
+//
+// Replaces the reference's flags.comp -> recursive scan of the flags in the element
+// type -> host read of the kept count -> scatter.comp schedule
+// (KernelLauncher::launch_compact, src/kernel/kernel_launcher.cpp:1701-1791; about
+// 9 host-synchronised dispatches and 7-9 n*s bytes, with float positions that lose
+// exactness above 2^24) with ONE launch: each tile evaluates the predicate on its
+// elements, ranks the kept ones with DPP wave scans of integer counts, stages them in
+// LDS in order, looks back for the number kept before the tile, and writes them as one
+// contiguous, coalesced run.  Bytes moved: n*s read + kept*s written.
+//
+// Partition needs the total kept count before any rejected element can be placed
+// (kept first, the rest after num_true, both stable): a counting pass writes num_true
+// to device memory and the compaction pass reads it there (no host round trip).
+#include "dev_common.hpp"
+#include "dev_pred.hpp"
+#include "dev_tiles.hpp"
+#include "kernels.hpp"
+
+namespace pxk {
+
+namespace {
+
+template <class T, int K, class C, int MODE, bool UNIQUE>
+__global__ __launch_bounds__(256) void k_compact(const T* in_al, int64_t in_shift, T* out,
+                                                 SegGeom g, px_desc pd, uint64_t* counts,
+                                                 const uint64_t* num_true_dev, uint64_t* status,
+                                                 Ctl* ctl) {
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int TILE = 256 * K * VEC;
+  using V = Vec<T, VEC>;
+  const V* vin = reinterpret_cast<const V*>(in_al);
+
+  __shared__ T s_stage[TILE];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_epoch;
+  __shared__ C s_w[4];
+  __shared__ C s_prefix;
+
+  const Pred<T> pred(pd);
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
+
+  while (true) {
+    if (threadIdx.x == 0)
+      s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t epoch = s_epoch;
+    if (tile >= g.ntiles) break;
+    const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
+
+    T x[K][VEC];
+    uint32_t fl[K];  // bit j: element j of chunk k is kept
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+      uint32_t valid = 0;
+      if (p0 < ti.hi && p0 + VEC > ti.lo) {
+        const V v = vin[p0 / VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          x[k][j] = v.v[j];
+          if (p0 + j >= ti.lo && p0 + j < ti.hi) valid |= 1u << j;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) x[k][j] = T(0);
+      }
+      uint32_t f = 0;
+      if constexpr (UNIQUE) {
+        // previous element of lane's first item: lane-1's last item, or memory for lane 0
+        T prev = dpp_mov<kWaveShr1, 0xf, 0xf>(x[k][VEC - 1]);
+        if (lane == 0 && p0 - 1 >= ti.lo && p0 - 1 < ti.hi) prev = in_al[p0 - 1];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const T pv = (j == 0) ? prev : x[k][j - 1];
+          const bool keep = (p0 + j == ti.lo) || !(x[k][j] == pv);
+          if (((valid >> j) & 1u) && keep) f |= 1u << j;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          if (((valid >> j) & 1u) && pred(x[k][j])) f |= 1u << j;
+      }
+      fl[k] = f;
+    }
+
+    // ---- rank kept elements: per-chunk wave scans of counts
+    C run = 0;
+    C tex[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const C c = static_cast<C>(__builtin_popcount(fl[k]));
+      const C incl = wave_incl_scan(c);
+      tex[k] = run + wave_shift_right1(incl);
+      run += read_lane(incl, kWave - 1);
+    }
+    if (lane == 0) s_w[wid] = run;
+    __syncthreads();
+    C wave_off = 0;
+    if (wid >= 1) wave_off = s_w[0];
+    if (wid >= 2) wave_off += s_w[1];
+    if (wid >= 3) wave_off += s_w[2];
+    const C tile_kept = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    const int64_t vstart = ti.w0 > ti.lo ? ti.w0 : ti.lo;  // first valid position of the tile
+    const int64_t vend = (ti.w0 + TILE) < ti.hi ? (ti.w0 + TILE) : ti.hi;
+    const C tile_valid = static_cast<C>(vend > vstart ? vend - vstart : 0);
+
+    // ---- stage kept (and, for partition, rejected) elements in LDS, in order
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      C r = wave_off + tex[k];
+      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const bool keep = (fl[k] >> j) & 1u;
+        if (keep) {
+          s_stage[r] = x[k][j];
+          ++r;
+        } else if (MODE == PX_SCATTER_PARTITION) {
+          const int64_t p = p0 + j;
+          if (p >= ti.lo && p < ti.hi) {
+            const C vi = static_cast<C>(p - vstart);  // valid index inside the tile
+            s_stage[tile_kept + (vi - r)] = x[k][j];
+          }
+        }
+      }
+    }
+
+    // ---- kept before this tile (look-back), then the tile's count
+    if (wid == 0) {
+      C prefix;
+      if (tile == ti.first_tile) {
+        prefix = 0;
+      } else {
+        if (lane == 0) publish<C>(status, tile, epoch, kStAgg, tile_kept);
+        prefix = lookback<C>(status, tile, ti.first_tile, epoch);
+      }
+      if (tile != ti.last_tile && lane == 0)
+        publish<C>(status, tile, epoch, kStIncl, prefix + tile_kept);
+      if (lane == 0) {
+        s_prefix = prefix;
+        if (tile == ti.last_tile) {
+          const uint64_t total = static_cast<uint64_t>(prefix) + tile_kept;
+          if (counts) counts[ti.seg] = total;
+          else ctl->result = total;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- coalesced writes of the staged run(s)
+    const uint64_t P = s_prefix;
+    const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
+    T* o = out + seg_out + P;
+    for (uint32_t j = threadIdx.x; j < tile_kept; j += 256) o[j] = s_stage[j];
+    if (MODE == PX_SCATTER_PARTITION) {
+      const uint64_t num_true = *num_true_dev;
+      const uint64_t rej_before = static_cast<uint64_t>(vstart - ti.lo) - P;
+      T* orj = out + seg_out + num_true + rej_before;
+      for (uint32_t j = tile_kept + threadIdx.x; j < tile_valid; j += 256)
+        orj[j - tile_kept] = s_stage[j];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_end(ctl, s_epoch);
+}
+
+// Predicate count over one range (partition's first pass): grid-stride, last block
+// writes the total.
+template <class T>
+__global__ __launch_bounds__(256) void k_count(const T* in, uint64_t n, px_desc pd,
+                                               uint64_t* partials, Ctl* ctl, uint64_t* out) {
+  const Pred<T> pred(pd);
+  __shared__ uint64_t s_w[4];
+  __shared__ int s_last;
+  uint64_t c = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * 256)
+    c += pred(in[i]) ? 1u : 0u;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  uint64_t w = wave_sum(c);
+  if (lane == 0) s_w[wid] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st_carry(&partials[blockIdx.x], s_w[0] + s_w[1] + s_w[2] + s_w[3]);
+    const uint32_t d =
+        __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (d == gridDim.x - 1) ? 1 : 0;
+    if (s_last) st_agent32(&ctl->done, 0u);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  uint64_t t = 0;
+  for (uint32_t j = threadIdx.x; j < gridDim.x; j += 256) t += ld_carry(&partials[j]);
+  w = wave_sum(t);
+  if (lane == 0) s_w[wid] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+template <class T, class C, int MODE, bool UNIQUE>
+void launch_compact_t(uint32_t grid, hipStream_t stream, const T* in_al, int64_t in_shift, T* out,
+                      const SegGeom& g, const px_desc& pd, uint64_t* counts,
+                      const uint64_t* num_true_dev, const Scratch& s) {
+  k_compact<T, kScanK, C, MODE, UNIQUE><<<grid, 256, 0, stream>>>(
+      in_al, in_shift, out, g, pd, counts, num_true_dev, s.status, s.ctl);
+}
+
+template <class T>
+hipError_t compact_t(const px_desc& pd, int mode, const void* in, void* out, const SegGeom& g,
+                     uint64_t* counts, const uint64_t* num_true_dev, const Scratch& s,
+                     hipStream_t stream) {
+  const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
+  const int64_t in_shift = elem_shift(in, sizeof(T));
+  const uint32_t grid = g.ntiles < s.max_blocks ? g.ntiles : s.max_blocks;
+  if (grid == 0) return hipSuccess;
+  // total element count decides the count width
+  uint64_t total = g.n;
+  const bool wide = (g.seg_lo != nullptr) ? false : (total >= (1ull << 32));
+  const size_t words = wide ? 2 : 1;
+  if (static_cast<size_t>(g.ntiles) * words > s.status_words) return hipErrorInvalidValue;
+  const bool uniq = (pd.fn == PX_PRED_UNIQUE);
+  T* o = reinterpret_cast<T*>(out);
+  if (mode == PX_SCATTER_PARTITION) {
+    if (g.seg_lo != nullptr || num_true_dev == nullptr) return hipErrorInvalidValue;
+    if (wide)
+      launch_compact_t<T, uint64_t, PX_SCATTER_PARTITION, false>(grid, stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
+    else
+      launch_compact_t<T, uint32_t, PX_SCATTER_PARTITION, false>(grid, stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
+  } else if (uniq) {
+    if (wide)
+      launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, true>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+    else
+      launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, true>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+  } else {
+    if (wide)
+      launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, false>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+    else
+      launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, false>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+  }
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t count_t(const px_desc& pd, const void* in, uint64_t n, uint64_t* count_dev,
+                   const Scratch& s, hipStream_t stream) {
+  uint64_t grid = (n + 256 * 16 - 1) / (256 * 16);
+  if (grid < 1) grid = 1;
+  if (grid > s.max_blocks) grid = s.max_blocks;
+  if (grid * sizeof(uint64_t) > s.partial_bytes) return hipErrorInvalidValue;
+  k_count<T><<<static_cast<uint32_t>(grid), 256, 0, stream>>>(
+      reinterpret_cast<const T*>(in), n, pd, reinterpret_cast<uint64_t*>(s.partials), s.ctl,
+      count_dev);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t compact(const px_desc& pred, int mode, const void* in, void* out, const SegGeom& g,
+                   uint64_t* counts, const uint64_t* num_true_dev, const Scratch& s,
+                   hipStream_t stream) {
+  PXK_DISPATCH(pred.dtype, return compact_t<T>(pred, mode, in, out, g, counts, num_true_dev, s, stream));
+  return hipErrorInvalidValue;
+}
+
+hipError_t count_if(const px_desc& pred, const void* in, uint64_t n, uint64_t* count_dev,
+                    const Scratch& s, hipStream_t stream) {
+  PXK_DISPATCH(pred.dtype, return count_t<T>(pred, in, n, count_dev, s, stream));
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pxk

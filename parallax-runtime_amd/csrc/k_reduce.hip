@@ -1,0 +1,215 @@
+// k_reduce.hip -- sum reduction for gfx950.
+//
+// Plain form (reference: KernelLauncher::launch_reduce, src/kernel/kernel_launcher.cpp:908-985,
+// which runs reduce.comp level after level with a host wait between levels): here ONE
+// launch.  A grid sized to fill every CU streams the input with 16-byte loads, four
+// in flight per thread, accumulating in registers in the carry type; each wave folds
+// with DPP, waves fold through LDS, every block leaves one partial, and the last block
+// to finish (agent-scope counter) folds the partials in a fixed order and writes the
+// result.  The counter resets itself, so back-to-back calls need no reset launch.
+//
+// Segmented form: a persistent grid walks the tiles of all segments (static
+// assignment: no tile waits on another); single-tile segments write their sum
+// directly, larger ones leave per-tile partials and the last tile of the segment (a
+// self-resetting per-segment counter) folds them in tile order.
+#include "dev_common.hpp"
+#include "dev_tiles.hpp"
+#include "kernels.hpp"
+
+namespace pxk {
+
+namespace {
+
+template <class T, int VEC>
+__global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int64_t lo,
+                                                int64_t hi,
+                                                typename Traits<T>::A* __restrict__ partials,
+                                                Ctl* ctl, T* out) {
+  using A = typename Traits<T>::A;
+  using V = Vec<T, VEC>;
+  const V* vin = reinterpret_cast<const V*>(in_al);
+  __shared__ A s_w[4];
+  __shared__ int s_last;
+
+  const int64_t a = (lo + VEC - 1) / VEC;  // interior vectors [a, b)
+  const int64_t b = hi / VEC;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  int64_t i = a + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  A acc = acc_zero<A>();
+  for (; i + 3 * stride < b; i += 4 * stride) {
+    const V v0 = vin[i];
+    const V v1 = vin[i + stride];
+    const V v2 = vin[i + 2 * stride];
+    const V v3 = vin[i + 3 * stride];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v0.v[j]));
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v1.v[j]));
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v2.v[j]));
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v3.v[j]));
+  }
+  for (; i < b; i += stride) {
+    const V v0 = vin[i];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v0.v[j]));
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2 * VEC) {
+    // head [lo, min(a*VEC, hi)) and tail [max(a,b)*VEC, hi): fewer than VEC each
+    int64_t pos;
+    bool ok;
+    if (threadIdx.x < VEC) {
+      pos = lo + threadIdx.x;
+      ok = pos < a * VEC && pos < hi;
+    } else {
+      pos = (a > b ? a : b) * VEC + (threadIdx.x - VEC);
+      ok = pos < hi;
+    }
+    if (ok) acc = acc_add(acc, to_acc(in_al[pos]));
+  }
+
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  A w = wave_sum(acc);
+  if (lane == 0) s_w[wid] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
+    st_carry(&partials[blockIdx.x], tot);
+    const uint32_t d =
+        __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (d == gridDim.x - 1) ? 1 : 0;
+    if (s_last) st_agent32(&ctl->done, 0u);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  A c = acc_zero<A>();
+  for (uint32_t j = threadIdx.x; j < gridDim.x; j += 256) c = acc_add(c, ld_carry(&partials[j]));
+  w = wave_sum(c);
+  if (lane == 0) s_w[wid] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
+    *out = from_acc<T>(tot);
+  }
+}
+
+template <class T, int K>
+__global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al, int64_t shift,
+                                                    SegGeom g,
+                                                    typename Traits<T>::A* __restrict__ partials,
+                                                    uint32_t* seg_cnt, T* __restrict__ out) {
+  using A = typename Traits<T>::A;
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int TILE = 256 * K * VEC;
+  using V = Vec<T, VEC>;
+  const V* vin = reinterpret_cast<const V*>(in_al);
+  __shared__ A s_w[4];
+  __shared__ int s_last;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+
+  for (uint32_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
+    const TileInfo ti = tile_info<VEC, TILE>(g, tile, shift);
+    A acc = acc_zero<A>();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t p0 = ti.w0 + (static_cast<int64_t>(k) * 256 + threadIdx.x) * VEC;
+      if (p0 < ti.hi && p0 + VEC > ti.lo) {
+        const V v = vin[p0 / VEC];
+        if (p0 >= ti.lo && p0 + VEC <= ti.hi) {
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v.v[j]));
+        } else {
+#pragma unroll
+          for (int j = 0; j < VEC; ++j)
+            if (p0 + j >= ti.lo && p0 + j < ti.hi) acc = acc_add(acc, to_acc(v.v[j]));
+        }
+      }
+    }
+    A w = wave_sum(acc);
+    if (lane == 0) s_w[wid] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
+      s_last = 0;
+      if (ti.first_tile == ti.last_tile) {
+        out[ti.seg] = from_acc<T>(tot);
+      } else {
+        st_carry(&partials[tile], tot);
+        const uint32_t d = __hip_atomic_fetch_add(&seg_cnt[ti.seg], 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        if (d == ti.last_tile - ti.first_tile) {
+          st_agent32(&seg_cnt[ti.seg], 0u);
+          s_last = 1;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_last) {
+      __threadfence();
+      A c = acc_zero<A>();
+      for (uint32_t j = ti.first_tile + threadIdx.x; j <= ti.last_tile; j += 256)
+        c = acc_add(c, ld_carry(&partials[j]));
+      w = wave_sum(c);
+      __syncthreads();
+      if (lane == 0) s_w[wid] = w;
+      __syncthreads();
+      if (threadIdx.x == 0)
+        out[ti.seg] = from_acc<T>(acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3])));
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+hipError_t reduce_t(const void* in, uint64_t n, void* out_dev, const Scratch& s,
+                    hipStream_t stream) {
+  using A = typename Traits<T>::A;
+  constexpr int VEC = 16 / sizeof(T);
+  const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
+  const int64_t lo = elem_shift(in, sizeof(T));
+  const int64_t hi = lo + static_cast<int64_t>(n);
+  const int64_t nvec = hi / VEC - (lo + VEC - 1) / VEC;
+  int64_t grid = (nvec + 1023) / 1024;
+  if (grid < 1) grid = 1;
+  const int64_t cap = static_cast<int64_t>(s.max_blocks);
+  if (grid > cap) grid = cap;
+  if (static_cast<size_t>(grid) * sizeof(A) > s.partial_bytes) return hipErrorInvalidValue;
+  k_reduce<T, VEC><<<static_cast<uint32_t>(grid), 256, 0, stream>>>(
+      in_al, lo, hi, reinterpret_cast<A*>(s.partials), s.ctl, reinterpret_cast<T*>(out_dev));
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t seg_reduce_t(const void* in, const SegGeom& g, void* out, const Scratch& s,
+                        hipStream_t stream) {
+  using A = typename Traits<T>::A;
+  const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
+  const int64_t shift = elem_shift(in, sizeof(T));
+  if (static_cast<size_t>(g.ntiles) * sizeof(A) > s.partial_bytes) return hipErrorInvalidValue;
+  if (g.nseg > s.seg_counters) return hipErrorInvalidValue;
+  uint32_t grid = g.ntiles < s.max_blocks ? g.ntiles : s.max_blocks;
+  if (grid == 0) return hipSuccess;
+  k_seg_reduce<T, kSegReduceK><<<grid, 256, 0, stream>>>(
+      in_al, shift, g, reinterpret_cast<A*>(s.partials), s.seg_cnt, reinterpret_cast<T*>(out));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t reduce(int dtype, const void* in, uint64_t n, void* out_dev, const Scratch& s,
+                  hipStream_t stream) {
+  PXK_DISPATCH(dtype, return reduce_t<T>(in, n, out_dev, s, stream));
+  return hipErrorInvalidValue;
+}
+
+hipError_t seg_reduce(int dtype, const void* in, const SegGeom& g, void* out, const Scratch& s,
+                      hipStream_t stream) {
+  PXK_DISPATCH(dtype, return seg_reduce_t<T>(in, g, out, s, stream));
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pxk

@@ -1,0 +1,184 @@
+// k_scan.hip -- single-pass inclusive / exclusive prefix scan for gfx950.
+//
+// Replaces the reference's scan.comp + scan_add.comp pair and its recursive launch
+// schedule (KernelLauncher::launch_scan, src/kernel/kernel_launcher.cpp:1280-1342: per-
+// block scan, recursive scan of block sums, add pass; about 4*n*s bytes in 7 host-
+// synchronised dispatches at 2^28) and the exclusive shift pass (launch_exclusive_scan,
+// :1437-1475) with ONE launch that reads every element once and writes it once (2*n*s).
+//
+// Tile = 256 threads x K chunks x one 16-byte vector per lane.  Wave w owns K
+// consecutive 64-vector chunks; per chunk each lane sums its vector, the wave scans the
+// 64 sums with DPP and carries the running total to the next chunk; the four wave
+// totals meet in LDS.  Tiles take tickets in order; each publishes its aggregate, wave 0
+// looks back over up to 64 predecessors per step (decoupled look-back), publishes its
+// inclusive prefix, and the tile writes its outputs with 16-byte stores.  Tiles of a
+// batched (segmented) call look back only inside their own segment.
+#include <cstring>
+
+#include "dev_common.hpp"
+#include "dev_tiles.hpp"
+#include "kernels.hpp"
+
+namespace pxk {
+
+namespace {
+
+template <class T, int K, bool EXCL>
+__global__ __launch_bounds__(256) void k_scan(const T* in_al, int64_t in_shift, T* out,
+                                              int64_t out_shift, SegGeom g,
+                                              typename Traits<T>::A init, uint64_t* status,
+                                              Ctl* ctl) {
+  using A = typename Traits<T>::A;
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int TILE = 256 * K * VEC;
+  using V = Vec<T, VEC>;
+  const V* vin = reinterpret_cast<const V*>(in_al);
+  V* vout = reinterpret_cast<V*>(reinterpret_cast<uintptr_t>(out) & ~uintptr_t(15));
+
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_epoch;
+  __shared__ A s_w[4];
+  __shared__ A s_prefix;
+
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
+
+  while (true) {
+    if (threadIdx.x == 0)
+      s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t epoch = s_epoch;
+    if (tile >= g.ntiles) break;
+    const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
+
+    // ---- load (masked outside the segment)
+    T x[K][VEC];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+      if (p0 >= ti.lo && p0 + VEC <= ti.hi) {
+        const V v = vin[p0 / VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) x[k][j] = v.v[j];
+      } else if (p0 < ti.hi && p0 + VEC > ti.lo) {
+        const V v = vin[p0 / VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) x[k][j] = (p0 + j >= ti.lo && p0 + j < ti.hi) ? v.v[j] : T(0);
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) x[k][j] = T(0);
+      }
+    }
+
+    // ---- per-chunk wave scans, carried across the wave's chunks
+    A run = acc_zero<A>();
+    A tex[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      A c = to_acc(x[k][0]);
+#pragma unroll
+      for (int j = 1; j < VEC; ++j) c = acc_add(c, to_acc(x[k][j]));
+      const A incl = wave_incl_scan(c);
+      const A tot = read_lane(incl, kWave - 1);
+      tex[k] = acc_add(run, wave_shift_right1(incl));
+      run = acc_add(run, tot);
+    }
+    if (lane == 0) s_w[wid] = run;
+    __syncthreads();
+
+    // ---- wave offsets and tile total in a fixed order
+    A wave_off = acc_zero<A>();
+    if (wid >= 1) wave_off = s_w[0];
+    if (wid >= 2) wave_off = acc_add(wave_off, s_w[1]);
+    if (wid >= 3) wave_off = acc_add(wave_off, s_w[2]);
+
+    if (wid == 0) {
+      const A tile_tot = acc_add(acc_add(acc_add(s_w[0], s_w[1]), s_w[2]), s_w[3]);
+      A prefix;
+      if (tile == ti.first_tile) {
+        prefix = EXCL ? init : acc_zero<A>();
+      } else {
+        if (lane == 0) publish<A>(status, tile, epoch, kStAgg, tile_tot);
+        prefix = lookback<A>(status, tile, ti.first_tile, epoch);
+      }
+      if (tile != ti.last_tile && lane == 0)
+        publish<A>(status, tile, epoch, kStIncl, acc_add(prefix, tile_tot));
+      if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+
+    // ---- outputs (16-byte stores when the output lines up with the input vectors)
+    const A base = acc_add(s_prefix, wave_off);
+    const int64_t odelta = ti.out_delta - in_shift;  // output element = position + odelta
+    const bool vec_ok = ((out_shift + odelta) & (VEC - 1)) == 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      A r = acc_add(base, tex[k]);
+      V y;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        if (EXCL) {
+          y.v[j] = from_acc<T>(r);
+          r = acc_add(r, to_acc(x[k][j]));
+        } else {
+          r = acc_add(r, to_acc(x[k][j]));
+          y.v[j] = from_acc<T>(r);
+        }
+      }
+      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+      if (vec_ok && p0 >= ti.lo && p0 + VEC <= ti.hi) {
+        vout[(p0 + odelta + out_shift) / VEC] = y;
+      } else if (p0 < ti.hi && p0 + VEC > ti.lo) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          if (p0 + j >= ti.lo && p0 + j < ti.hi) out[p0 + j + odelta] = y.v[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_end(ctl, s_epoch);
+}
+
+template <class T>
+hipError_t scan_t(const void* in, void* out, const SegGeom& g, bool exclusive, const void* init,
+                  const Scratch& s, hipStream_t stream) {
+  using A = typename Traits<T>::A;
+  const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
+  const int64_t in_shift = elem_shift(in, sizeof(T));
+  const int64_t out_shift = elem_shift(out, sizeof(T));
+  if (static_cast<size_t>(g.ntiles) * (sizeof(A) / 4) > s.status_words) return hipErrorInvalidValue;
+  A iv{};
+  if (exclusive && init) {
+    T t;
+    std::memcpy(&t, init, sizeof(T));
+    if constexpr (sizeof(A) == 16) {
+      double h = static_cast<double>(t);
+      std::memcpy(&iv, &h, sizeof(double));  // dd{t, 0}
+    } else if constexpr (Traits<T>::is_float) {
+      iv = static_cast<A>(t);
+    } else {
+      iv = static_cast<A>(t);
+    }
+  }
+  const uint32_t grid = g.ntiles < s.max_blocks ? g.ntiles : s.max_blocks;
+  if (grid == 0) return hipSuccess;
+  if (exclusive)
+    k_scan<T, kScanK, true><<<grid, 256, 0, stream>>>(in_al, in_shift, reinterpret_cast<T*>(out),
+                                                       out_shift, g, iv, s.status, s.ctl);
+  else
+    k_scan<T, kScanK, false><<<grid, 256, 0, stream>>>(in_al, in_shift, reinterpret_cast<T*>(out),
+                                                        out_shift, g, iv, s.status, s.ctl);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t scan(int dtype, const void* in, void* out, const SegGeom& g, bool exclusive,
+                const void* init, const Scratch& s, hipStream_t stream) {
+  PXK_DISPATCH(dtype, return scan_t<T>(in, out, g, exclusive, init, s, stream));
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pxk

@@ -1,0 +1,374 @@
+// k_sort.hip -- bitonic sort for gfx950.
+//
+// The reference dispatches one global compare-exchange stage per (k, j) of the bitonic
+// schedule with a host wait after each (KernelLauncher::launch_sort,
+// src/kernel/kernel_launcher.cpp:1568-1606, bitonic.comp:15-30: 351 synchronous full
+// passes at 2^26).  This engine sorts the same network with far fewer HBM passes:
+//
+//   * the network is written in its all-ascending form: merge level k starts with the
+//     "flip" stage (partner i ^ (k-1)) followed by half-cleaners (partner i ^ j,
+//     j = k/4 .. 1); every comparator puts the minimum at the lower index.  Positions
+//     at or past n therefore always keep the +infinity value they start with, so they
+//     are virtual: never loaded, never stored, and any n is accepted (the reference
+//     needs a power of two and its funnel pads with numeric_limits::max in memory,
+//     stdpar.hpp:159-176).
+//   * every stage whose partner distance is below the LDS tile runs inside one kernel
+//     with the tile resident in LDS (the tile sort, then one merge pass per level);
+//   * the remaining (global) stages of a level run C at a time in registers: a thread
+//     loads the 2^C elements that the C stages exchange among themselves, applies the
+//     C stages, and stores them back -- one HBM pass for up to 5 stages.
+//
+// Segmented form: the same kernels walk a block table, so thousands of segments of
+// different lengths sort in the same few launches (each segment virtually padded to its
+// own power of two).
+#include <algorithm>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "kernels.hpp"
+
+namespace pxk {
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ T pos_inf();
+template <>
+__device__ __forceinline__ int32_t pos_inf<int32_t>() { return INT32_MAX; }
+template <>
+__device__ __forceinline__ uint32_t pos_inf<uint32_t>() { return UINT32_MAX; }
+template <>
+__device__ __forceinline__ int64_t pos_inf<int64_t>() { return INT64_MAX; }
+template <>
+__device__ __forceinline__ uint64_t pos_inf<uint64_t>() { return UINT64_MAX; }
+template <>
+__device__ __forceinline__ float pos_inf<float>() { return __builtin_huge_valf(); }
+template <>
+__device__ __forceinline__ double pos_inf<double>() { return __builtin_huge_val(); }
+
+// min to a, max to b; a swap only when b < a (operator<, as std::sort)
+template <class T>
+__device__ __forceinline__ void cmpx(T& a, T& b) {
+  if (b < a) {
+    const T t = a;
+    a = b;
+    b = t;
+  }
+}
+
+constexpr int kLocalThreads = 512;
+
+// Segment of the current block: plain (one segment) or from the block table.
+struct SortSeg {
+  uint64_t base, n;
+  uint32_t nplog;   // log2 of the segment's padded length
+  uint32_t lblock;  // block index inside the segment
+};
+
+__device__ __forceinline__ uint32_t ceil_log2(uint64_t n) {
+  return n <= 1 ? 0u : 64u - static_cast<uint32_t>(__clzll(static_cast<long long>(n - 1)));
+}
+
+// Segment bounds travel as one array: seg[2*s] = first element, seg[2*s+1] = end.
+__device__ __forceinline__ SortSeg sort_seg(const uint64_t* seg_off, const uint32_t* blk_seg,
+                                            const uint32_t* blk_first, uint64_t n) {
+  SortSeg s;
+  if (seg_off == nullptr) {
+    s.base = 0;
+    s.n = n;
+    s.lblock = blockIdx.x;
+  } else {
+    const uint32_t sg = blk_seg[blockIdx.x];
+    s.base = seg_off[2 * sg];
+    s.n = seg_off[2 * sg + 1] - s.base;
+    s.lblock = blockIdx.x - blk_first[sg];
+  }
+  s.nplog = ceil_log2(s.n);
+  return s;
+}
+
+// LDS kernel.  MERGE == false: full sort of each tile (levels 1 .. tile).  MERGE == true:
+// the half-cleaners j < tile of level `kl` (> tile).
+template <class T, int TILE_LOG, bool MERGE>
+__global__ __launch_bounds__(kLocalThreads) void k_sort_local(T* data, uint64_t n,
+                                                              const uint64_t* seg_off,
+                                                              const uint32_t* blk_seg,
+                                                              const uint32_t* blk_first,
+                                                              uint32_t kl) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  T* s = reinterpret_cast<T*>(s_raw);
+  const SortSeg sg = sort_seg(seg_off, blk_seg, blk_first, n);
+  const uint32_t tl = sg.nplog < TILE_LOG ? sg.nplog : TILE_LOG;
+  const uint32_t te = 1u << tl;
+  const uint64_t tbase = static_cast<uint64_t>(sg.lblock) << tl;
+  if (tbase >= sg.n || tl == 0) return;
+  if (MERGE && sg.nplog < kl) return;  // segment already sorted below this level
+  T* d = data + sg.base + tbase;
+  const uint64_t lim = sg.n - tbase;  // valid elements of this tile (may exceed te)
+
+  for (uint32_t i = threadIdx.x; i < te; i += kLocalThreads) s[i] = i < lim ? d[i] : pos_inf<T>();
+  __syncthreads();
+  const uint32_t pairs = te >> 1;
+  if (!MERGE) {
+    for (uint32_t lv = 1; lv <= tl; ++lv) {
+      const uint32_t hk = 1u << (lv - 1);
+      for (uint32_t p = threadIdx.x; p < pairs; p += kLocalThreads) {
+        const uint32_t i = ((p >> (lv - 1)) << lv) | (p & (hk - 1));
+        const uint32_t q = i ^ ((hk << 1) - 1);
+        cmpx(s[i], s[q]);
+      }
+      __syncthreads();
+      for (int jl = static_cast<int>(lv) - 2; jl >= 0; --jl) {
+        const uint32_t j = 1u << jl;
+        for (uint32_t p = threadIdx.x; p < pairs; p += kLocalThreads) {
+          const uint32_t i = ((p >> jl) << (jl + 1)) | (p & (j - 1));
+          cmpx(s[i], s[i + j]);
+        }
+        __syncthreads();
+      }
+    }
+  } else {
+    for (int jl = static_cast<int>(tl) - 1; jl >= 0; --jl) {
+      const uint32_t j = 1u << jl;
+      for (uint32_t p = threadIdx.x; p < pairs; p += kLocalThreads) {
+        const uint32_t i = ((p >> jl) << (jl + 1)) | (p & (j - 1));
+        cmpx(s[i], s[i + j]);
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < te; i += kLocalThreads)
+    if (i < lim) d[i] = s[i];
+}
+
+// Register pass: C consecutive stages of level kl whose top partner bit is b (the
+// group's bits are b-C+1 .. b).  FLIP: the first stage is the level's flip stage.
+template <class T, int C, bool FLIP>
+__global__ __launch_bounds__(256) void k_sort_global(T* data, uint64_t n, const uint64_t* seg_off,
+                                                     const uint32_t* blk_seg,
+                                                     const uint32_t* blk_first, uint32_t b,
+                                                     uint32_t kl) {
+  constexpr int Q = 1 << C;
+  const SortSeg sg = sort_seg(seg_off, blk_seg, blk_first, n);
+  if (sg.nplog < kl) return;  // segment too short for this level
+  const uint64_t groups = (1ull << sg.nplog) >> C;
+  const uint64_t gid = static_cast<uint64_t>(sg.lblock) * 256 + threadIdx.x;
+  if (gid >= groups) return;
+  const uint32_t lowb = b - C + 1;
+  const uint64_t irep = ((gid >> lowb) << (b + 1)) | (gid & ((1ull << lowb) - 1));
+  T* d = data + sg.base;
+  const uint64_t lim = sg.n;
+
+  T v[Q];
+  uint64_t e[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    uint64_t x = 0;
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+      if ((q >> (C - 1 - r)) & 1) {
+        const uint64_t m = (FLIP && r == 0) ? ((1ull << kl) - 1) : (1ull << (b - r));
+        x ^= m;
+      }
+    }
+    e[q] = irep ^ x;
+    v[q] = e[q] < lim ? d[e[q]] : pos_inf<T>();
+  }
+#pragma unroll
+  for (int r = 0; r < C; ++r) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if ((q >> (C - 1 - r)) & 1) continue;
+      const int q2 = q | (1 << (C - 1 - r));
+      // The element of q sits at the lower index unless the flip mask (which sets every
+      // bit below b) was applied to it and this stage's bit is one of those bits.
+      const bool q_high = FLIP && r >= 1 && ((q >> (C - 1)) & 1);
+      if (!q_high)
+        cmpx(v[q], v[q2]);
+      else
+        cmpx(v[q2], v[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    if (e[q] < lim) d[e[q]] = v[q];
+}
+
+template <class T>
+struct SortCfg {
+  static constexpr int kTileLog = (sizeof(T) == 4) ? 14 : 13;  // 64 KiB of LDS
+  static constexpr int kMaxC = 5;
+};
+
+template <class T>
+hipError_t launch_local(bool merge, uint32_t blocks, T* data, uint64_t n, const uint64_t* seg_off,
+                        const uint32_t* blk_seg, const uint32_t* blk_first, uint32_t kl,
+                        hipStream_t stream) {
+  constexpr int TL = SortCfg<T>::kTileLog;
+  const size_t lds = sizeof(T) << TL;
+  if (merge)
+    k_sort_local<T, TL, true><<<blocks, kLocalThreads, lds, stream>>>(data, n, seg_off, blk_seg, blk_first, kl);
+  else
+    k_sort_local<T, TL, false><<<blocks, kLocalThreads, lds, stream>>>(data, n, seg_off, blk_seg, blk_first, kl);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_global(int c, bool flip, uint32_t blocks, T* data, uint64_t n,
+                         const uint64_t* seg_off, const uint32_t* blk_seg,
+                         const uint32_t* blk_first, uint32_t b, uint32_t kl, hipStream_t stream) {
+#define PXK_G(CC)                                                                                \
+  case CC:                                                                                       \
+    if (flip)                                                                                    \
+      k_sort_global<T, CC, true><<<blocks, 256, 0, stream>>>(data, n, seg_off, blk_seg, blk_first, b, kl); \
+    else                                                                                         \
+      k_sort_global<T, CC, false><<<blocks, 256, 0, stream>>>(data, n, seg_off, blk_seg, blk_first, b, kl); \
+    break;
+  switch (c) {
+    PXK_G(1)
+    PXK_G(2)
+    PXK_G(3)
+    PXK_G(4)
+    PXK_G(5)
+    default: return hipErrorInvalidValue;
+  }
+#undef PXK_G
+  return hipGetLastError();
+}
+
+inline uint32_t host_ceil_log2(uint64_t n) {
+  uint32_t l = 0;
+  while ((1ull << l) < n) ++l;
+  return l;
+}
+
+template <class T>
+hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream) {
+  T* data = reinterpret_cast<T*>(data_v);
+  if (n <= 1) return hipSuccess;
+  constexpr int TL = SortCfg<T>::kTileLog;
+  const uint32_t nplog = host_ceil_log2(n);
+  const uint32_t tl = nplog < TL ? nplog : TL;
+  // tiles that hold at least one real element
+  const uint64_t ntiles = (n + (1ull << tl) - 1) >> tl;
+  hipError_t e = launch_local<T>(false, static_cast<uint32_t>(ntiles), data, n, nullptr, nullptr,
+                                 nullptr, 0, stream);
+  if (e != hipSuccess) return e;
+  for (uint32_t kl = tl + 1; kl <= nplog; ++kl) {
+    int remaining = static_cast<int>(kl - tl);
+    uint32_t b = kl - 1;
+    bool first = true;
+    while (remaining > 0) {
+      const int c = std::min(remaining, SortCfg<T>::kMaxC);
+      const uint64_t groups = (1ull << nplog) >> c;
+      const uint64_t blocks = (groups + 255) / 256;
+      e = launch_global<T>(c, first, static_cast<uint32_t>(blocks), data, n, nullptr, nullptr,
+                           nullptr, b, kl, stream);
+      if (e != hipSuccess) return e;
+      b -= c;
+      remaining -= c;
+      first = false;
+    }
+    e = launch_local<T>(true, static_cast<uint32_t>(ntiles), data, n, nullptr, nullptr, nullptr, kl,
+                        stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// ---- segmented: block tables per launch kind, built on the host.
+//   local (tile sort and merges): blocks(s) = ceil(n_s / tile_s)
+//   global pass with c stages:    blocks(s) = ceil((np_s >> c) / 256), only for segments
+//                                 longer than one tile (others never reach a global stage)
+template <class T>
+bool build_plan(const uint64_t* lo, const uint64_t* hi, uint32_t nseg, SegSortPlan& p) {
+  constexpr int TL = SortCfg<T>::kTileLog;
+  constexpr int MC = SortCfg<T>::kMaxC;
+  p.nseg = nseg;
+  p.tile_log = TL;
+  p.max_c = MC;
+  p.max_nplog = 0;
+  std::vector<uint32_t> local_seg, local_first(nseg, 0);
+  std::vector<std::vector<uint32_t>> gseg(MC + 1), gfirst(MC + 1, std::vector<uint32_t>(nseg, 0));
+  for (uint32_t s = 0; s < nseg; ++s) {
+    if (hi[s] < lo[s]) return false;
+    const uint64_t len = hi[s] - lo[s];
+    const uint32_t nl = host_ceil_log2(len);
+    p.max_nplog = std::max(p.max_nplog, nl);
+    const uint32_t tl = nl < static_cast<uint32_t>(TL) ? nl : static_cast<uint32_t>(TL);
+    const uint64_t blocks = len <= 1 ? 0 : (len + (1ull << tl) - 1) >> tl;
+    local_first[s] = static_cast<uint32_t>(local_seg.size());
+    for (uint64_t k = 0; k < blocks; ++k) local_seg.push_back(s);
+    for (int c = 1; c <= MC; ++c) {
+      gfirst[c][s] = static_cast<uint32_t>(gseg[c].size());
+      if (nl <= static_cast<uint32_t>(TL)) continue;
+      const uint64_t gb = (((1ull << nl) >> c) + 255) / 256;
+      for (uint64_t k = 0; k < gb; ++k) gseg[c].push_back(s);
+    }
+  }
+  p.tables.clear();
+  p.tables.insert(p.tables.end(), local_seg.begin(), local_seg.end());
+  p.nlocal = static_cast<uint32_t>(local_seg.size());
+  p.o_local_first = p.tables.size();
+  p.tables.insert(p.tables.end(), local_first.begin(), local_first.end());
+  for (int c = 1; c <= MC; ++c) {
+    p.o_gseg[c] = p.tables.size();
+    p.gblocks[c] = static_cast<uint32_t>(gseg[c].size());
+    p.tables.insert(p.tables.end(), gseg[c].begin(), gseg[c].end());
+    p.o_gfirst[c] = p.tables.size();
+    p.tables.insert(p.tables.end(), gfirst[c].begin(), gfirst[c].end());
+  }
+  return true;
+}
+
+template <class T>
+hipError_t seg_sort_run_t(const SegSortPlan& p, void* data_v, const uint64_t* off_d,
+                          const uint32_t* w, hipStream_t stream) {
+  T* data = reinterpret_cast<T*>(data_v);
+  constexpr int TL = SortCfg<T>::kTileLog;
+  if (p.nlocal == 0) return hipSuccess;
+  hipError_t e = launch_local<T>(false, p.nlocal, data, 0, off_d, w, w + p.o_local_first, 0, stream);
+  if (e != hipSuccess) return e;
+  for (uint32_t kl = TL + 1; kl <= p.max_nplog; ++kl) {
+    int remaining = static_cast<int>(kl - TL);
+    uint32_t b = kl - 1;
+    bool first = true;
+    while (remaining > 0) {
+      const int c = std::min(remaining, SortCfg<T>::kMaxC);
+      if (p.gblocks[c]) {
+        e = launch_global<T>(c, first, p.gblocks[c], data, 0, off_d, w + p.o_gseg[c],
+                             w + p.o_gfirst[c], b, kl, stream);
+        if (e != hipSuccess) return e;
+      }
+      b -= c;
+      remaining -= c;
+      first = false;
+    }
+    // merge pass: blocks of segments shorter than this level return at once
+    e = launch_local<T>(true, p.nlocal, data, 0, off_d, w, w + p.o_local_first, kl, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+bool seg_sort_plan(int dtype, const uint64_t* lo_host, const uint64_t* hi_host, uint32_t nseg,
+                   SegSortPlan& p) {
+  p.dtype = dtype;
+  PXK_DISPATCH(dtype, return build_plan<T>(lo_host, hi_host, nseg, p));
+  return false;
+}
+
+hipError_t seg_sort_run(const SegSortPlan& p, void* data, const uint64_t* seg_off_dev,
+                        const uint32_t* tables_dev, hipStream_t stream) {
+  PXK_DISPATCH(p.dtype, return seg_sort_run_t<T>(p, data, seg_off_dev, tables_dev, stream));
+  return hipErrorInvalidValue;
+}
+
+hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream) {
+  PXK_DISPATCH(dtype, return sort_t<T>(data, n, stream));
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pxk

@@ -1,0 +1,160 @@
+// kernels.hpp -- host-callable launchers of the gfx950 primitive kernels.
+//
+// Every launcher is stream-ordered and allocation-free: the caller (a stream context)
+// owns the scratch below and guarantees its sizes.  Pointers are device pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "parallax/descriptor.h"
+
+namespace pxk {
+
+// Control block shared by the self-cleaning kernels of one stream context.  Zeroed once
+// at allocation with epoch = 1; every launch leaves ticket == done == 0 and advances
+// the epoch, so no reset launch or memset is needed between calls (and graph replays
+// stay valid).
+struct Ctl {
+  uint32_t epoch;   // current launch epoch (1 .. 2^30-1)
+  uint32_t ticket;  // dynamic tile ticket
+  uint32_t done;    // blocks finished in this launch
+  uint32_t pad0;
+  uint64_t result;  // kept count written by the last compaction tile
+  uint64_t pad1[5];
+  uint8_t value[64];  // reduce result slot (one element, written by the last block)
+};
+
+constexpr uint32_t kEpochMax = (1u << 30) - 1;
+
+struct SegGeom;
+
+// Tile sizes (items per thread = K * 16 / sizeof(T)); the autotuner may pick others
+// for the segmented forms through the *_k variants.
+constexpr int kSegReduceK = 4;
+constexpr int kScanK = 4;
+
+// Scratch of one stream context.  All of it is device memory; ctl is zeroed once with
+// epoch = 1 and is left clean by every kernel.
+struct Scratch {
+  Ctl* ctl = nullptr;
+  uint64_t* status = nullptr;   // look-back status words
+  size_t status_words = 0;
+  void* partials = nullptr;     // per-block / per-tile partial sums
+  size_t partial_bytes = 0;
+  uint32_t* seg_cnt = nullptr;  // per-segment counters (self-resetting)
+  size_t seg_counters = 0;
+  uint32_t max_blocks = 2048;   // resident-block cap (CUs x 8)
+};
+
+#define PXK_DISPATCH(dtype, ...)                                 \
+  switch (dtype) {                                               \
+    case PX_I32: { using T = int32_t; __VA_ARGS__; } break;      \
+    case PX_U32: { using T = uint32_t; __VA_ARGS__; } break;     \
+    case PX_I64: { using T = int64_t; __VA_ARGS__; } break;      \
+    case PX_U64: { using T = uint64_t; __VA_ARGS__; } break;     \
+    case PX_F32: { using T = float; __VA_ARGS__; } break;        \
+    case PX_F64: { using T = double; __VA_ARGS__; } break;       \
+    default: break;                                              \
+  }
+
+inline size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case PX_I32: case PX_U32: case PX_F32: return 4;
+    case PX_I64: case PX_U64: case PX_F64: return 8;
+    default: return 0;
+  }
+}
+// Bytes of the carry type (status words per tile = carry_bytes / 4).
+inline size_t carry_bytes(int dtype) {
+  switch (dtype) {
+    case PX_I32: case PX_U32: return 4;
+    case PX_I64: case PX_U64: case PX_F32: return 8;
+    case PX_F64: return 16;
+    default: return 0;
+  }
+}
+// Elements per tile of the tiled kernels for a given K.
+inline uint32_t tile_elems(int dtype, int k) {
+  return static_cast<uint32_t>(256 * k * (16 / dtype_size(dtype)));
+}
+
+// ---- reduce (k_reduce.hip)
+hipError_t reduce(int dtype, const void* in, uint64_t n, void* out_dev, const Scratch& s,
+                  hipStream_t stream);
+hipError_t seg_reduce(int dtype, const void* in, const SegGeom& g, void* out, const Scratch& s,
+                      hipStream_t stream);
+
+// ---- scan (k_scan.hip): inclusive, or exclusive with `init` (host bytes of one element)
+hipError_t scan(int dtype, const void* in, void* out, const SegGeom& g, bool exclusive,
+                const void* init, const Scratch& s, hipStream_t stream);
+
+// ---- compaction (k_compact.hip)
+//   mode PX_SCATTER_COMPACT: kept elements of segment s to out + seg_out[s] (or
+//     seg_lo[s]); kept count to counts[s] (counts may be null for one segment: then
+//     Ctl::result).
+//   mode PX_SCATTER_PARTITION (one segment): kept first, the rest after *num_true_dev.
+hipError_t compact(const px_desc& pred, int mode, const void* in, void* out, const SegGeom& g,
+                   uint64_t* counts, const uint64_t* num_true_dev, const Scratch& s,
+                   hipStream_t stream);
+// Count of predicate-true elements of one segment into *count_dev (partition's first pass).
+hipError_t count_if(const px_desc& pred, const void* in, uint64_t n, uint64_t* count_dev,
+                    const Scratch& s, hipStream_t stream);
+
+// ---- sort (k_sort.hip): ascending; positions past n act as +infinity.
+hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream);
+// Segmented sort of [seg_off[s], seg_off[s+1]) for every segment: a host plan (block
+// tables for each launch kind) uploaded by the caller, then a fixed launch sequence.
+struct SegSortPlan {
+  int dtype = 0;
+  uint32_t nseg = 0;
+  std::vector<uint32_t> tables;  // local_seg | local_first | per c: glob_seg[c] | glob_first[c]
+  size_t o_local_first = 0;
+  size_t o_gseg[8] = {};
+  size_t o_gfirst[8] = {};
+  uint32_t nlocal = 0;
+  uint32_t gblocks[8] = {};
+  uint32_t max_nplog = 0;
+  uint32_t tile_log = 0;
+  int max_c = 0;
+};
+// seg_bounds_dev: 2*nseg device words, [first, end) element indices of each segment.
+bool seg_sort_plan(int dtype, const uint64_t* lo_host, const uint64_t* hi_host, uint32_t nseg,
+                   SegSortPlan& p);
+hipError_t seg_sort_run(const SegSortPlan& p, void* data, const uint64_t* seg_bounds_dev,
+                        const uint32_t* tables_dev, hipStream_t stream);
+
+// ---- element-wise (k_map.hip)
+struct MapArgs {
+  uint32_t count_lo;   // push block {count, pad, host_base, dev_base} (24 bytes)
+  uint32_t pad;
+  uint64_t host_base;  // arena host mirror base (relocation of captured pointers)
+  uint64_t dev_base;   // arena device base
+  uint64_t arena_bytes;
+  uint8_t cap[64];     // capture bytes (first 64)
+  uint32_t cap_size;
+};
+hipError_t transform(const px_desc& d, const void* in, void* out, uint64_t n, const MapArgs& a,
+                     hipStream_t stream);
+// Rewrite n stored host addresses inside [host_base, host_base+bytes) to device addresses.
+hipError_t relocate_pointers(uint64_t* ptrs, uint64_t n, uint64_t host_base, uint64_t dev_base,
+                             uint64_t bytes, hipStream_t stream);
+// The reference's relocation probe: r = dev_base + (host_ptr - host_base); r[i] = r[i]*2+1.
+hipError_t relocate_probe(uint64_t host_ptr, uint64_t host_base, uint64_t dev_base,
+                          uint32_t count, hipStream_t stream);
+
+// ---- search (k_search.hip): results written to *out_dev (uint64 index)
+hipError_t find_first(const px_desc& pred, const void* in, uint64_t n, uint64_t* out_dev,
+                      const Scratch& s, hipStream_t stream);
+hipError_t argminmax(int dtype, const void* in, uint64_t n, bool want_max, bool want_last,
+                     uint64_t* out_dev, const Scratch& s, hipStream_t stream);
+hipError_t mismatch(int dtype, const void* a, const void* b, uint64_t n, uint64_t* out_dev,
+                    const Scratch& s, hipStream_t stream);
+
+// ---- fill kernel (graph-safe memset replacement)
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
+
+}  // namespace pxk

@@ -1,0 +1,275 @@
+"""Parity of the HIP primitives (device-resident entry points) with the oracle, for all
+six element types, tile-boundary sizes, misaligned pointers and in-place calls.
+
+Integer, sort, compaction and unique results must be bit-exact.  Float reduce and scan
+must satisfy |err| <= 2*ceil(log2 n)*eps*sum|x| against the double-double oracle (eps =
+machine epsilon of the element type; for scans sum|x| is taken over each prefix)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import pxo
+from parallax_amd import lib, check, make_desc, consts as C
+from parallax_amd.device import DeviceBuffer
+from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_I64, PX_U32, PX_U64, dtype_code
+
+pytestmark = pytest.mark.gpu
+
+ALL = [PX_I32, PX_U32, PX_I64, PX_U64, PX_F32, PX_F64]
+SIZES = [1, 2, 3, 7, 64, 255, 256, 257, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 65537, 1_000_003]
+FLOATS = (PX_F32, PX_F64)
+
+
+def gen(dt, n, seed, lo=None, hi=None):
+    rng = np.random.default_rng(seed)
+    if dt in FLOATS:
+        return (rng.standard_normal(n) * 3.0).astype(NP_DTYPES[dt])
+    if lo is None:
+        lo = -(1 << 30) if dt in (PX_I32, PX_I64) else 0
+        hi = 1 << 30
+    return rng.integers(lo, hi, n).astype(NP_DTYPES[dt])
+
+
+def bound(dt, n):
+    eps = np.finfo(NP_DTYPES[dt]).eps
+    return 2 * max(1, math.ceil(math.log2(max(n, 2)))) * eps
+
+
+def upload(a, shift=0):
+    """Device copy of `a` starting `shift` elements past a 256-byte boundary."""
+    b = DeviceBuffer(a.nbytes + shift * a.itemsize + 64)
+    b.upload(a, shift * a.itemsize)
+    return b, b.addr(shift * a.itemsize)
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", SIZES)
+def test_reduce(gpu_ctx, dt, n):
+    a = gen(dt, n, 100 + n + dt)
+    for shift in (0, 1):
+        buf, p = upload(a, shift)
+        out = DeviceBuffer(16)
+        check(lib.pxd_reduce(gpu_ctx.h, dt, p, n, out.ptr), "reduce")
+        gpu_ctx.sync()
+        got = out.download(a.dtype, 1)[0]
+        if dt in FLOATS:
+            hi, lo, ab = pxo.sum_exact(a)
+            err = abs(float(got) - (hi + lo))
+            assert err <= bound(dt, n) * ab + 1e-300, (n, err, ab)
+        else:
+            assert got == pxo.reduce(a), (n, shift)
+
+
+def test_reduce_empty_writes_zero(gpu_ctx):
+    out = DeviceBuffer.from_array(np.array([123], dtype=np.int64))
+    check(lib.pxd_reduce(gpu_ctx.h, PX_I64, out.ptr, 0, out.ptr), "reduce")
+    gpu_ctx.sync()
+    assert out.download(np.int64, 1)[0] == 0
+
+
+def check_scan(dt, a, got, exclusive, init=0):
+    if dt in FLOATS:
+        val, ab = pxo.scan_exact(a, exclusive, init)
+        err = np.abs(got.astype(np.float64) - val)
+        assert np.all(err <= bound(dt, a.size) * ab + 1e-300), float((err / np.maximum(ab, 1e-300)).max())
+    else:
+        ref = pxo.exclusive_scan(a, init) if exclusive else pxo.inclusive_scan(a)
+        assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", SIZES)
+def test_inclusive_scan(gpu_ctx, dt, n):
+    a = gen(dt, n, 200 + n + dt)
+    ib, ip = upload(a, 0)
+    ob, op = upload(np.zeros_like(a), 0)
+    check(lib.pxd_inclusive_scan(gpu_ctx.h, dt, ip, op, n), "scan")
+    gpu_ctx.sync()
+    check_scan(dt, a, ob.download(a.dtype, n), False)
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", [1, 5, 4097, 100_003])
+@pytest.mark.parametrize("shifts", [(1, 1), (1, 2), (3, 0)])
+def test_scan_misaligned(gpu_ctx, dt, n, shifts):
+    a = gen(dt, n, 300 + n + dt)
+    ib, ip = upload(a, shifts[0])
+    ob, op = upload(np.zeros_like(a), shifts[1])
+    check(lib.pxd_inclusive_scan(gpu_ctx.h, dt, ip, op, n), "scan")
+    gpu_ctx.sync()
+    check_scan(dt, a, ob.download(a.dtype, n, shifts[1] * a.itemsize), False)
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", [1, 4096, 70001])
+def test_exclusive_scan_with_init_and_in_place(gpu_ctx, dt, n):
+    a = gen(dt, n, 400 + n + dt)
+    init = np.array([5], dtype=a.dtype)
+    b, p = upload(a, 0)
+    check(lib.pxd_exclusive_scan(gpu_ctx.h, dt, p, p, n, init.ctypes.data), "xscan")
+    gpu_ctx.sync()
+    check_scan(dt, a, b.download(a.dtype, n), True, init[0])
+
+
+def test_scan_u32_many_tiles_lookback_window(gpu_ctx):
+    # more than 64 predecessors per look-back window, small values, wraps checked exactly
+    a = gen(PX_U32, (1 << 22) + 13, 7, 0, 16)
+    b, p = upload(a)
+    o, q = upload(np.zeros_like(a))
+    for _ in range(3):  # repeated launches on one context exercise the epoch / ticket reset
+        check(lib.pxd_inclusive_scan(gpu_ctx.h, PX_U32, p, q, a.size), "scan")
+    gpu_ctx.sync()
+    assert np.array_equal(o.download(np.uint32, a.size), np.cumsum(a, dtype=np.uint32))
+
+
+SORT_SIZES = [0, 1, 2, 3, 17, 1000, 1024, 4096, 8191, 8192, 8193, 16383, 16384, 16385, 70000, (1 << 20) + 3]
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", SORT_SIZES)
+def test_sort(gpu_ctx, dt, n):
+    a = gen(dt, n, 500 + n + dt)
+    if n > 10:
+        a[::7] = a[0]  # duplicates
+    b, p = upload(a, 1 if n % 2 else 0)
+    check(lib.pxd_sort(gpu_ctx.h, dt, p, n), "sort")
+    gpu_ctx.sync()
+    got = b.download(a.dtype, n, (1 if n % 2 else 0) * a.itemsize)
+    assert got.tobytes() == pxo.sort(a).tobytes()
+
+
+def test_sort_extremes(gpu_ctx):
+    a = np.array([np.inf, -np.inf, 0.0, 1e300, -1e-300, np.inf, 3.0] * 3000, dtype=np.float64)
+    b, p = upload(a)
+    check(lib.pxd_sort(gpu_ctx.h, PX_F64, p, a.size), "sort")
+    gpu_ctx.sync()
+    assert b.download(np.float64, a.size).tobytes() == np.sort(a).tobytes()
+    u = np.array([0xFFFFFFFF, 0, 0xFFFFFFFF, 5] * 5001, dtype=np.uint32)
+    b, p = upload(u)
+    check(lib.pxd_sort(gpu_ctx.h, PX_U32, p, u.size), "sort")
+    gpu_ctx.sync()
+    assert np.array_equal(b.download(np.uint32, u.size), np.sort(u))
+
+
+PREDS = [
+    (C.PRED_GT, 0, 0, 0), (C.PRED_LT, 100, 0, C.FLAG_NEGATE), (C.PRED_MOD_EQ, 2, 0, 0),
+    (C.PRED_IN_RANGE, -1000, 1000, 0), (C.PRED_NONZERO, 0, 0, 0), (C.PRED_TRUE, 0, 0, 0),
+    (C.PRED_EQ, 12345, 0, 0),
+]
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("pred", PREDS)
+@pytest.mark.parametrize("n", [1, 4097, 300_001])
+def test_copy_if(gpu_ctx, dt, pred, n):
+    fn, c0, c1, fl = pred
+    if dt in (PX_U32, PX_U64) and c0 < 0:
+        c0 = 0
+    a = gen(dt, n, 600 + n + dt, *((None, None) if dt in FLOATS else ((-5000, 5000) if dt in (PX_I32, PX_I64) else (0, 5000))))
+    d = make_desc(C.OP_FLAGS, dt, fn=fn, flags=fl, c0=c0, c1=c1)
+    ib, ip = upload(a, 1)
+    sentinel = np.full(n, 7, dtype=a.dtype)
+    ob, op = upload(sentinel)
+    cnt = DeviceBuffer(8)
+    check(lib.pxd_copy_if(gpu_ctx.h, ctypes.byref(d), ip, op, n, cnt.ptr), "copy_if")
+    gpu_ctx.sync()
+    k = int(cnt.download(np.uint64, 1)[0])
+    ref = pxo.copy_if(d, a)
+    assert k == ref.size
+    got = ob.download(a.dtype, n)
+    assert got[:k].tobytes() == ref.tobytes()
+    assert np.all(got[k:] == 7)  # nothing written past the kept count
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", [1, 2, 4097, 200_003])
+def test_unique(gpu_ctx, dt, n):
+    a = np.sort(gen(dt, n, 700 + dt, *((None, None) if dt in FLOATS else (0, 50))))
+    if dt in FLOATS:
+        a = np.round(a, 1).astype(a.dtype)
+    d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_UNIQUE)
+    ib, ip = upload(a, 3)
+    ob, op = upload(np.zeros_like(a))
+    cnt = DeviceBuffer(8)
+    check(lib.pxd_copy_if(gpu_ctx.h, ctypes.byref(d), ip, op, n, cnt.ptr), "unique")
+    gpu_ctx.sync()
+    k = int(cnt.download(np.uint64, 1)[0])
+    ref = pxo.unique(a)
+    assert k == ref.size and ob.download(a.dtype, k).tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", [1, 4097, 300_001])
+def test_partition_stable(gpu_ctx, dt, n):
+    a = gen(dt, n, 800 + n + dt)
+    d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=0 if dt not in (PX_U32, PX_U64) else 1 << 29)
+    ib, ip = upload(a)
+    ob, op = upload(np.zeros_like(a))
+    nt = DeviceBuffer(8)
+    check(lib.pxd_partition(gpu_ctx.h, ctypes.byref(d), ip, op, n, nt.ptr), "partition")
+    gpu_ctx.sync()
+    k = int(nt.download(np.uint64, 1)[0])
+    sref, sk = pxo.partition(d, a, stable=True)
+    ref, rk = pxo.partition(d, a)
+    got = ob.download(a.dtype, n)
+    assert k == sk == rk
+    assert got.tobytes() == sref.tobytes()  # the GPU's order is the stable one
+    assert sorted(got[:k].tolist()) == sorted(ref[:k].tolist())  # same sides as std::partition
+
+
+MAPS = [(C.MAP_IDENTITY, 0, 0), (C.MAP_FILL, 9, 0), (C.MAP_ADD, 3, 0), (C.MAP_MUL, -2, 0),
+        (C.MAP_AXPB, 3, -1), (C.MAP_NEG, 0, 0), (C.MAP_ABS, 0, 0), (C.MAP_SQUARE, 0, 0)]
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("mp", MAPS)
+def test_transform_same_type(gpu_ctx, dt, mp):
+    fn, c0, c1 = mp
+    if dt in (PX_U32, PX_U64) and c0 < 0:
+        c0 = 2
+    a = gen(dt, 10007, 900 + dt, *((None, None) if dt in FLOATS else (-1000, 1000) if dt in (PX_I32, PX_I64) else (0, 1000)))
+    d = make_desc(C.OP_TRANSFORM, dt, aux=dt, fn=fn, c0=c0, c1=c1)
+    ib, ip = upload(a)
+    ob, op = upload(np.zeros_like(a))
+    check(lib.pxd_transform(gpu_ctx.h, ctypes.byref(d), ip, op, a.size, None, 0), "transform")
+    gpu_ctx.sync()
+    assert ob.download(a.dtype, a.size).tobytes() == pxo.transform(d, a, a.dtype).tobytes()
+
+
+@pytest.mark.parametrize("pair", [(PX_F32, PX_F64), (PX_I32, PX_I64), (PX_I64, PX_F64), (PX_U32, PX_F32)])
+def test_transform_converting_with_captures(gpu_ctx, pair):
+    tin, tout = pair
+    a = gen(tin, 5000, 950, *((None, None) if tin in FLOATS else (-100, 100) if tin in (PX_I32, PX_I64) else (0, 100)))
+    d = make_desc(C.OP_TRANSFORM, tin, aux=tout, fn=C.MAP_AXPB)
+    cap = np.array([2, 1], dtype=NP_DTYPES[tout]).tobytes()
+    ib, ip = upload(a)
+    ob, op = upload(np.zeros(a.size, dtype=NP_DTYPES[tout]))
+    check(lib.pxd_transform(gpu_ctx.h, ctypes.byref(d), ip, op, a.size, cap, len(cap)), "transform")
+    gpu_ctx.sync()
+    ref = pxo.transform(d, a, NP_DTYPES[tout], cap)
+    assert ob.download(NP_DTYPES[tout], a.size).tobytes() == ref.tobytes()
+
+
+def test_search_entry_points(gpu_ctx):
+    a = gen(PX_I32, 100_000, 960, -1000, 1000)
+    a[77_777] = 5000
+    a[88_888] = 5000
+    a[1234] = -5000
+    ib, ip = upload(a)
+    idx = DeviceBuffer(8)
+    d = make_desc(C.OP_FIND, PX_I32, fn=C.PRED_GT, c0=4000)
+    check(lib.pxd_find(gpu_ctx.h, ctypes.byref(d), ip, a.size, idx.ptr), "find")
+    gpu_ctx.sync()
+    assert idx.download(np.uint64, 1)[0] == 77_777
+    for want_max, want_last, expect in [(1, 0, 77_777), (1, 1, 88_888), (0, 0, 1234)]:
+        check(lib.pxd_argminmax(gpu_ctx.h, PX_I32, ip, a.size, want_max, want_last, idx.ptr), "argminmax")
+        gpu_ctx.sync()
+        assert idx.download(np.uint64, 1)[0] == expect
+    b = a.copy()
+    b[99_000] += 1
+    bb, bp = upload(b)
+    check(lib.pxd_mismatch(gpu_ctx.h, PX_I32, ip, bp, a.size, idx.ptr), "mismatch")
+    gpu_ctx.sync()
+    assert idx.download(np.uint64, 1)[0] == 99_000

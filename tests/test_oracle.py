@@ -1,0 +1,213 @@
+"""The oracle pinned against the reference's known-answer tests, closed forms and brute
+force (CPU only).  Each check would fail for a plausible mistake in the oracle: a
+dropped element, an off-by-one in the exclusive scan, a reversed comparator, an unstable
+compaction, a wrong remainder convention."""
+import math
+
+import numpy as np
+import pytest
+
+import pxo
+from parallax_amd import make_desc, consts as C
+from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_I64, PX_U32, PX_U64
+
+ALL = [PX_I32, PX_U32, PX_I64, PX_U64, PX_F32, PX_F64]
+
+
+# ---- reference known answers (tests/unit/test_*.cpp of the reference)
+def test_reduce_known_answer_5000_mod8():
+    # test_reduce.cpp:47-63 and test_staging.cpp:54-66: data[i] = i % 8, N = 5000 -> 17500
+    a = (np.arange(5000) % 8).astype(np.float32)
+    assert pxo.reduce(a) == np.float32(17500.0)
+
+
+def test_scan_known_answer_ones():
+    # test_scan.cpp:46-61: inclusive scan of 1000 ones is 1..1000
+    a = np.ones(1000, dtype=np.float32)
+    assert np.array_equal(pxo.inclusive_scan(a), np.arange(1, 1001, dtype=np.float32))
+
+
+def test_sort_known_answer_permutation():
+    # test_sort.cpp:41-68: v = ((N-1-i) * 2654435761) % N is a permutation of 0..N-1
+    N = 1024
+    v = np.array([((N - 1 - i) * 2654435761) % N for i in range(N)], dtype=np.float32)
+    assert np.array_equal(pxo.sort(v), np.arange(N, dtype=np.float32))
+
+
+def test_copy_if_known_answer_odd():
+    # test_copy_if.cpp:54-74: in[i] = i odd ? i : 0, keep > 0.5 -> 500 kept, out[j] = 2j+1
+    i = np.arange(1000)
+    a = np.where(i % 2 == 1, i, 0).astype(np.float32)
+    d = make_desc(C.OP_FLAGS, PX_F32, fn=C.PRED_GT, c0=0.5)
+    out = pxo.copy_if(d, a)
+    assert out.size == 500 and np.array_equal(out, (2 * np.arange(500) + 1).astype(np.float32))
+
+
+# ---- closed forms and brute force
+@pytest.mark.parametrize("dt", ALL)
+def test_reduce_integer_closed_form(dt):
+    n = 10007
+    a = (np.arange(n) % 97).astype(NP_DTYPES[dt])
+    expect = sum(int(x) for x in a)
+    assert int(pxo.reduce(a)) == expect
+
+
+def test_reduce_wraps_modulo():
+    a = np.full(3, 2**31 - 1, dtype=np.int32)
+    assert int(pxo.reduce(a)) == ((3 * (2**31 - 1) + 2**31) % 2**32) - 2**31
+
+
+def test_reduce_is_sequential_float_fold():
+    # the fallback is a left fold in float32: (1e8 + 1) + ... loses the ones
+    a = np.array([1e8] + [1.0] * 10, dtype=np.float32)
+    acc = np.float32(0)
+    for x in a:
+        acc = np.float32(acc + x)
+    assert pxo.reduce(a) == acc and acc == np.float32(1e8)
+
+
+def test_sum_exact_matches_fsum():
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal(20000) * 10.0 ** rng.integers(-8, 8, 20000)).astype(np.float64)
+    hi, lo, ab = pxo.sum_exact(a)
+    assert hi == math.fsum(a.tolist())
+    assert ab == pytest.approx(math.fsum(abs(x) for x in a.tolist()), rel=1e-15)
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_scans_brute_force(dt):
+    rng = np.random.default_rng(dt)
+    a = rng.integers(0, 50, 3001).astype(NP_DTYPES[dt])
+    acc, inc, exc = 0, [], []
+    for x in a.tolist():
+        exc.append(acc + 7)
+        acc += x
+        inc.append(acc)
+    assert np.array_equal(pxo.inclusive_scan(a), np.array(inc).astype(a.dtype))
+    assert np.array_equal(pxo.exclusive_scan(a, 7), np.array(exc).astype(a.dtype))
+
+
+def test_scan_exact_prefix():
+    a = np.array([1.0, 1e16, 1.0, -1e16, 3.0])
+    val, ab = pxo.scan_exact(a)
+    assert val.tolist() == [math.fsum(a[: i + 1].tolist()) for i in range(5)]
+    assert ab[-1] == math.fsum(abs(x) for x in a.tolist())
+    ve, _ = pxo.scan_exact(a, exclusive=True, init=2.0)
+    assert ve.tolist() == [math.fsum([2.0] + a[:i].tolist()) for i in range(5)]
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_sort_matches_sorted(dt):
+    rng = np.random.default_rng(10 + dt)
+    a = rng.integers(0, 1000, 5000).astype(NP_DTYPES[dt])
+    assert np.array_equal(pxo.sort(a), np.array(sorted(a.tolist())).astype(a.dtype))
+
+
+def _brute_pred(fn, x, c0, c1, neg):
+    if fn == C.PRED_TRUE: r = True
+    elif fn == C.PRED_EQ: r = x == c0
+    elif fn == C.PRED_NE: r = x != c0
+    elif fn == C.PRED_LT: r = x < c0
+    elif fn == C.PRED_LE: r = x <= c0
+    elif fn == C.PRED_GT: r = x > c0
+    elif fn == C.PRED_GE: r = x >= c0
+    elif fn == C.PRED_IN_RANGE: r = c0 <= x < c1
+    elif fn == C.PRED_MOD_EQ:
+        if isinstance(x, float):
+            r = math.fmod(x, c0) == c1
+        elif c0 == 0:
+            r = False
+        else:
+            q = abs(x) // abs(c0)
+            rem = x - (q if (x >= 0) == (c0 > 0) else -q) * c0  # C++ truncated remainder
+            r = rem == c1
+    elif fn == C.PRED_NONZERO: r = x != 0
+    elif fn == C.PRED_BITS_ANY: r = (int(x) & int(c0)) != 0
+    else: r = False
+    return (not r) if neg else r
+
+
+@pytest.mark.parametrize("fn", [C.PRED_TRUE, C.PRED_EQ, C.PRED_NE, C.PRED_LT, C.PRED_LE, C.PRED_GT,
+                                C.PRED_GE, C.PRED_IN_RANGE, C.PRED_MOD_EQ, C.PRED_NONZERO])
+@pytest.mark.parametrize("dt", [PX_I32, PX_U64, PX_F32, PX_F64])
+@pytest.mark.parametrize("neg", [0, 1])
+def test_predicates_brute_force(fn, dt, neg):
+    npd = NP_DTYPES[dt]
+    signed = dt in (PX_I32, PX_I64, PX_F32, PX_F64)
+    xs = [-7, -3, -1, 0, 1, 2, 3, 5, 9] if signed else [0, 1, 2, 3, 5, 9, 10]
+    c0 = 3 if fn != C.PRED_MOD_EQ else (-3 if signed else 3)
+    c1 = 5 if fn == C.PRED_IN_RANGE else (-1 if (fn == C.PRED_MOD_EQ and signed) else 1)
+    d = make_desc(C.OP_FLAGS, dt, fn=fn, flags=neg, c0=c0, c1=c1)
+    for x in xs:
+        xv = float(x) if dt in (PX_F32, PX_F64) else int(x)
+        expect = _brute_pred(fn, xv, float(c0) if isinstance(xv, float) else c0,
+                             float(c1) if isinstance(xv, float) else c1, neg)
+        assert pxo.pred_eval(d, npd.type(x)) == int(expect), (fn, dt, x)
+
+
+def test_copy_if_is_stable_and_remove_if_complements():
+    rng = np.random.default_rng(5)
+    a = rng.integers(-100, 100, 4000).astype(np.int64)
+    d = make_desc(C.OP_FLAGS, PX_I64, fn=C.PRED_MOD_EQ, c0=3, c1=0)
+    kept = pxo.copy_if(d, a)
+    trunc = np.fmod(a, 3) == 0  # C++ truncated remainder (numpy % is floored)
+    assert np.array_equal(kept, a[trunc])
+    assert np.array_equal(pxo.remove_if(d, a), a[~trunc])
+    assert pxo.count_if(d, a) == int(trunc.sum())
+
+
+def test_partition_point_and_sides():
+    rng = np.random.default_rng(6)
+    a = rng.integers(0, 1000, 3333).astype(np.int32)
+    d = make_desc(C.OP_FLAGS, PX_I32, fn=C.PRED_LT, c0=400)
+    out, k = pxo.partition(d, a)
+    sout, sk = pxo.partition(d, a, stable=True)
+    assert k == sk == int((a < 400).sum())
+    assert np.all(out[:k] < 400) and np.all(out[k:] >= 400)
+    assert sorted(out[:k].tolist()) == sorted(a[a < 400].tolist())
+    assert np.array_equal(sout[:sk], a[a < 400]) and np.array_equal(sout[sk:], a[a >= 400])
+
+
+def test_unique_keeps_first_of_runs():
+    a = np.array([1, 1, 2, 2, 2, 3, 1, 1, 4], dtype=np.int32)
+    assert pxo.unique(a).tolist() == [1, 2, 3, 1, 4]
+    f = np.array([0.0, -0.0, 0.0, np.nan, np.nan, 1.0], dtype=np.float64)
+    u = pxo.unique(f)  # -0.0 == 0.0 collapses; NaN != NaN keeps both
+    assert u.size == 4 and np.isnan(u[1]) and np.isnan(u[2])
+    ud = make_desc(C.OP_FLAGS, PX_I32, fn=C.PRED_UNIQUE)
+    assert pxo.copy_if(ud, a).tolist() == [1, 2, 3, 1, 4]
+
+
+def test_maps_and_transforms():
+    a = np.array([-3, -1, 0, 2, 5], dtype=np.int32)
+    assert pxo.map_inplace(make_desc(C.OP_MAP, PX_I32, fn=C.MAP_AXPB, c0=3, c1=-2), a).tolist() == \
+        [3 * x - 2 for x in a.tolist()]
+    assert pxo.map_inplace(make_desc(C.OP_MAP, PX_I32, fn=C.MAP_ABS), a).tolist() == [3, 1, 0, 2, 5]
+    cap = np.array([9], dtype=np.int32).tobytes()
+    assert pxo.map_inplace(make_desc(C.OP_MAP, PX_I32, fn=C.MAP_FILL), a, cap).tolist() == [9] * 5
+    f = np.array([0.1, 1.5, -2.25], dtype=np.float32)
+    d = make_desc(C.OP_TRANSFORM, PX_F32, aux=PX_F64, fn=C.MAP_MUL, c0=2.0, const_dtype=PX_F64)
+    assert pxo.transform(d, f, np.float64).tolist() == [float(x) * 2.0 for x in f.tolist()]
+    # fma: a*x+b rounded once (1 + 2^-30)*(1 - 2^-30) - 1 = -2^-60 exactly
+    x = np.array([1.0 - 2.0**-30])
+    d = make_desc(C.OP_MAP, PX_F64, fn=C.MAP_AXPB, c0=1.0 + 2.0**-30, c1=-1.0)
+    assert pxo.map_inplace(d, x)[0] == -(2.0**-60)
+    pd = make_desc(C.OP_TRANSFORM, PX_I32, aux=PX_I32, fn=C.MAP_PRED, fn2=C.PRED_GT, c0=0)
+    assert pxo.transform(pd, a, np.int32).tolist() == [0, 0, 0, 1, 1]
+
+
+def test_segmented_equal_per_segment_plain():
+    rng = np.random.default_rng(8)
+    lens = [0, 1, 5, 300, 0, 4097]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    a = rng.integers(0, 20, int(off[-1])).astype(np.int64)
+    r = pxo.seg_reduce(a, off)
+    s = pxo.seg_inclusive_scan(a, off)
+    so = pxo.seg_sort(a, off)
+    uq, uo = pxo.seg_unique(so, off)
+    for i in range(len(lens)):
+        seg = a[off[i]:off[i + 1]]
+        assert r[i] == pxo.reduce(seg)
+        assert np.array_equal(s[off[i]:off[i + 1]], pxo.inclusive_scan(seg))
+        assert np.array_equal(so[off[i]:off[i + 1]], np.sort(seg))
+        assert np.array_equal(uq[uo[i]:uo[i + 1]], pxo.unique(np.sort(seg)))
