@@ -11,7 +11,8 @@
 // Segmented form: a persistent grid walks the tiles of all segments (static
 // assignment: no tile waits on another); single-tile segments write their sum
 // directly, larger ones leave per-tile partials and the last tile of the segment (a
-// self-resetting per-segment counter) folds them in tile order.
+// self-resetting per-segment counter) folds them in tile order.  Segment s's sum goes
+// to out[seg_out[s]] (out[s] without an output table).
 #include "dev_common.hpp"
 #include "dev_tiles.hpp"
 #include "kernels.hpp"
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
       const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
       s_last = 0;
       if (ti.first_tile == ti.last_tile) {
-        out[ti.seg] = from_acc<T>(tot);
+        out[g.seg_out ? g.seg_out[ti.seg] : ti.seg] = from_acc<T>(tot);
       } else {
         st_carry(&partials[tile], tot);
         const uint32_t d = __hip_atomic_fetch_add(&seg_cnt[ti.seg], 1u, __ATOMIC_ACQ_REL,
@@ -158,7 +159,8 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
       if (lane == 0) s_w[wid] = w;
       __syncthreads();
       if (threadIdx.x == 0)
-        out[ti.seg] = from_acc<T>(acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3])));
+        out[g.seg_out ? g.seg_out[ti.seg] : ti.seg] =
+            from_acc<T>(acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3])));
     }
     __syncthreads();
   }

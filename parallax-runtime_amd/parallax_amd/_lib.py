@@ -98,6 +98,41 @@ _proto("pxb_plan_launch", _i, _vp)
 _proto("pxb_plan_destroy", None, _vp)
 
 
+# ---- parallax/px_serve.h
+class PxsConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("max_batch", ctypes.c_int), ("max_wait_us", ctypes.c_double)]
+
+
+class PxsRequest(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int), ("dtype", ctypes.c_int), ("n", ctypes.c_size_t),
+                ("d_in", ctypes.c_void_p), ("flags", PxDesc)]
+
+
+class PxsStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("first_arrival_s", ctypes.c_double), ("last_completion_s", ctypes.c_double),
+                ("req_per_s", ctypes.c_double), ("pool_bytes", ctypes.c_size_t),
+                ("pool_peak_bytes", ctypes.c_size_t)]
+
+
+_proto("pxs_config_default", None, ctypes.POINTER(PxsConfig))
+_proto("pxs_create", _vp, ctypes.POINTER(PxsConfig))
+_proto("pxs_destroy", None, _vp)
+_proto("pxs_alloc", _vp, _vp, _sz)
+_proto("pxs_free", None, _vp, _vp)
+_proto("pxs_submit", ctypes.c_int64, _vp, ctypes.POINTER(PxsRequest))
+_proto("pxs_wait", _i, _vp, ctypes.c_int64, ctypes.c_double)
+_proto("pxs_result", _i, _vp, ctypes.c_int64, ctypes.POINTER(_vp), _u64p, ctypes.POINTER(ctypes.c_double))
+_proto("pxs_release", _i, _vp, ctypes.c_int64)
+_proto("pxs_drain", _i, _vp)
+_proto("pxs_get_stats", _i, _vp, ctypes.POINTER(PxsStats))
+_proto("pxs_reset_stats", _i, _vp)
+
+
 def last_error():
     buf = ctypes.create_string_buffer(512)
     code = lib.pxr_last_error(buf, 512)

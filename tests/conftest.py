@@ -33,7 +33,7 @@ def has_gpu():
 def gpu_ctx():
     """One stream context for the whole GPU session (tests run in one process)."""
     if os.environ.get("PARALLAX_FORCE_CPU"):
-        pytest.skip("PARALLAX_FORCE_CPU is set")
+        pytest.fail("GPU test running with PARALLAX_FORCE_CPU set (it would measure nothing)")
     from parallax_amd.device import Context, device_count
     if device_count() == 0:
         pytest.fail("GPU test collected on a machine without a GPU")

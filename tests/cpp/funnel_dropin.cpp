@@ -1,0 +1,92 @@
+// funnel_dropin.cpp -- a user program written against parallax/stdpar.hpp, as the
+// reference's funnels are used (std::vector data, parallax::par policy).  Without a GPU
+// (or with PARALLAX_FORCE_CPU=1) every lookup misses and the ISO loops run; on the GPU
+// the fixed-function funnels resolve through the runtime's built-in table and copy_if
+// through a descriptor registered under its funnel key.  Prints one PASS/FAIL line per
+// check and the device launch counter; exit status 0 iff every check passed.
+#include <parallax/descriptor.h>
+#include <parallax/px_device.h>
+#include <parallax/stdpar.hpp>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+struct IsEven {
+  bool operator()(int x) const { return x % 2 == 0; }
+};
+
+static int failures = 0;
+static void check(bool ok, const char* what) {
+  std::printf("%s: %s\n", ok ? "PASS" : "FAIL", what);
+  if (!ok) ++failures;
+}
+
+int main() {
+  // the descriptor a compiler registrar would emit for device_copy_if<int, IsEven>
+  static px_desc flags{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_FLAGS, PX_I32, 0, PX_PRED_MOD_EQ, 0, 0, 2, 0};
+  static px_desc scan{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_SCAN, PX_I32, 0, 0, 0, 0, 0, 0};
+  static px_desc add{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_SCAN_ADD, PX_I32, 0, 0, 0, 0, 0, 0};
+  static px_desc scatter{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_SCATTER, PX_I32, PX_SCATTER_COMPACT, 0, 0, 0, 0, 0};
+  const char* key =
+      "std::size_t parallax::detail::device_copy_if(const T*, T*, std::size_t, Pred) "
+      "[with T = int; Pred = IsEven; std::size_t = long unsigned int]";
+  const std::string k(key);
+  parallax_kernel_register((k + ":flags").c_str(), reinterpret_cast<unsigned*>(&flags), PX_DESC_WORDS);
+  parallax_kernel_register((k + ":scan").c_str(), reinterpret_cast<unsigned*>(&scan), PX_DESC_WORDS);
+  parallax_kernel_register((k + ":add").c_str(), reinterpret_cast<unsigned*>(&add), PX_DESC_WORDS);
+  parallax_kernel_register((k + ":scatter").c_str(), reinterpret_cast<unsigned*>(&scatter), PX_DESC_WORDS);
+
+  std::mt19937_64 rng(7);
+  const size_t n = 1000003;
+  std::vector<int> vi(n);
+  for (auto& x : vi) x = static_cast<int>(rng() % 2001) - 1000;
+  std::vector<double> vd(n);
+  for (auto& x : vd) x = static_cast<double>(rng() % 100000) / 7.0;
+
+  // reduce
+  long long want = 0;
+  for (int x : vi) want += x;
+  check(parallax::reduce(parallax::par, vi.begin(), vi.end(), 0) == static_cast<int>(want), "reduce int");
+  const double sd = parallax::reduce(parallax::par, vd.begin(), vd.end(), 0.0);
+  double exact = 0;
+  for (double x : vd) exact += x;
+  check(std::fabs(sd - exact) <= 1e-9 * std::fabs(exact), "reduce double");
+  // scans
+  std::vector<int> inc(n), exc(n);
+  parallax::inclusive_scan(parallax::par, vi.begin(), vi.end(), inc.begin());
+  std::vector<int> inc_ref(n);
+  std::inclusive_scan(vi.begin(), vi.end(), inc_ref.begin());
+  check(inc == inc_ref, "inclusive_scan");
+  parallax::exclusive_scan(parallax::par, vi.begin(), vi.end(), exc.begin(), 5);
+  std::vector<int> exc_ref(n);
+  std::exclusive_scan(vi.begin(), vi.end(), exc_ref.begin(), 5);
+  check(exc == exc_ref, "exclusive_scan");
+  // sort (non power of two) then unique
+  std::vector<int> s = vi, s_ref = vi;
+  parallax::sort(parallax::par, s.begin(), s.end());
+  std::sort(s_ref.begin(), s_ref.end());
+  check(s == s_ref, "sort");
+  auto e = parallax::unique(parallax::par, s.begin(), s.end());
+  auto e_ref = std::unique(s_ref.begin(), s_ref.end());
+  check((e - s.begin()) == (e_ref - s_ref.begin()) && std::equal(s.begin(), e, s_ref.begin()), "unique");
+  // copy_if with a registered functor
+  std::vector<int> out(n), out_ref;
+  auto oe = parallax::copy_if(parallax::par, vi.begin(), vi.end(), out.begin(), IsEven{});
+  std::copy_if(vi.begin(), vi.end(), std::back_inserter(out_ref), IsEven{});
+  check(static_cast<size_t>(oe - out.begin()) == out_ref.size() &&
+            std::equal(out_ref.begin(), out_ref.end(), out.begin()), "copy_if IsEven");
+  // a capturing lambda has no descriptor: it stays on the CPU and is still correct
+  const int t = 17;
+  const long c = parallax::count_if(parallax::par, vi.begin(), vi.end(), [t](int x) { return x > t; });
+  check(c == std::count_if(vi.begin(), vi.end(), [t](int x) { return x > t; }), "count_if (host loop)");
+
+  uint64_t st[7] = {0};
+  pxr_stats(st);
+  std::printf("device_launches %llu\n", static_cast<unsigned long long>(st[0]));
+  return failures == 0 ? 0 : 1;
+}

@@ -1,0 +1,338 @@
+"""Batched (segmented) forms, graph plans and the request server, against the oracle
+applied per segment; plus "alone vs in the mix" identity."""
+import ctypes
+import threading
+
+import numpy as np
+import pytest
+
+import pxo
+from parallax_amd import lib, check, make_desc, consts as C, datagen
+from parallax_amd._lib import PxsConfig, PxsRequest, PxsStats
+from parallax_amd.device import Context, DeviceBuffer
+from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_I64, PX_U32, PX_U64
+
+pytestmark = pytest.mark.gpu
+
+ALL = [PX_I32, PX_U32, PX_I64, PX_U64, PX_F32, PX_F64]
+FLOATS = (PX_F32, PX_F64)
+
+
+def seg_input(dt, seed, nseg=300, lo=1, hi=70_000):
+    lens = datagen.log_uniform_lengths(seed, nseg, lo, hi)
+    lens[::37] = 0  # empty segments
+    lens[5] = 1
+    off = datagen.offsets_from_lengths(lens)
+    rng = np.random.default_rng(seed)
+    n = int(off[-1])
+    if dt in FLOATS:
+        a = rng.standard_normal(n).astype(NP_DTYPES[dt])
+    else:
+        a = rng.integers(0, 1000, n).astype(NP_DTYPES[dt])
+    return a, off
+
+
+def u64p(off):
+    return off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def bound(dt, n):
+    import math
+    return 2 * max(1, math.ceil(math.log2(max(n, 2)))) * float(np.finfo(NP_DTYPES[dt]).eps)
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_seg_reduce(gpu_ctx, dt):
+    a, off = seg_input(dt, 10 + dt)
+    d = DeviceBuffer.from_array(a)
+    out = DeviceBuffer(8 * (off.size - 1))
+    check(lib.pxb_seg_reduce(gpu_ctx.h, dt, d.ptr, u64p(off), off.size - 1, out.ptr), "seg_reduce")
+    gpu_ctx.sync()
+    got = out.download(a.dtype, off.size - 1)
+    if dt in FLOATS:
+        hi, ab = pxo.seg_sum_exact(a, off)
+        lens = np.diff(off.astype(np.int64))
+        for s in range(off.size - 1):
+            assert abs(float(got[s]) - hi[s]) <= bound(dt, lens[s]) * ab[s] + 1e-300, s
+    else:
+        assert np.array_equal(got, pxo.seg_reduce(a, off))
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_seg_scan(gpu_ctx, dt):
+    a, off = seg_input(dt, 20 + dt)
+    d = DeviceBuffer.from_array(a)
+    o = DeviceBuffer(a.nbytes + 16)
+    check(lib.pxb_seg_inclusive_scan(gpu_ctx.h, dt, d.ptr, o.ptr, u64p(off), off.size - 1), "seg_scan")
+    gpu_ctx.sync()
+    got = o.download(a.dtype, a.size)
+    if dt in FLOATS:
+        for s in range(off.size - 1):
+            seg = a[off[s]:off[s + 1]]
+            if seg.size == 0:
+                continue
+            val, ab = pxo.scan_exact(seg)
+            err = np.abs(got[off[s]:off[s + 1]].astype(np.float64) - val)
+            assert np.all(err <= bound(dt, seg.size) * ab + 1e-300), s
+    else:
+        assert np.array_equal(got, pxo.seg_inclusive_scan(a, off))
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_seg_sort(gpu_ctx, dt):
+    a, off = seg_input(dt, 30 + dt, nseg=200, lo=1, hi=300_000)
+    d = DeviceBuffer.from_array(a)
+    check(lib.pxb_seg_sort(gpu_ctx.h, dt, d.ptr, u64p(off), off.size - 1), "seg_sort")
+    gpu_ctx.sync()
+    assert d.download(a.dtype, a.size).tobytes() == pxo.seg_sort(a, off).tobytes()
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_seg_copy_if_and_unique(gpu_ctx, dt):
+    a, off = seg_input(dt, 40 + dt)
+    nseg = off.size - 1
+    d = DeviceBuffer.from_array(a)
+    o = DeviceBuffer(a.nbytes + 16)
+    cnt = DeviceBuffer(8 * nseg)
+    c0 = 0.0 if dt in FLOATS else 500
+    fl = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=c0)
+    check(lib.pxb_seg_copy_if(gpu_ctx.h, ctypes.byref(fl), d.ptr, o.ptr, u64p(off), nseg, cnt.ptr), "seg_copy_if")
+    gpu_ctx.sync()
+    got = o.download(a.dtype, a.size)
+    counts = cnt.download(np.uint64, nseg)
+    for s in range(nseg):
+        ref = pxo.copy_if(fl, a[off[s]:off[s + 1]])
+        assert counts[s] == ref.size
+        assert got[off[s]:off[s] + ref.size].tobytes() == ref.tobytes()
+    # unique after a segmented sort (the B2 shape)
+    srt = pxo.seg_sort(a, off)
+    d.upload(srt)
+    ud = make_desc(C.OP_FLAGS, dt, fn=C.PRED_UNIQUE)
+    check(lib.pxb_seg_copy_if(gpu_ctx.h, ctypes.byref(ud), d.ptr, o.ptr, u64p(off), nseg, cnt.ptr), "seg_unique")
+    gpu_ctx.sync()
+    got = o.download(a.dtype, a.size)
+    counts = cnt.download(np.uint64, nseg)
+    uq, uo = pxo.seg_unique(srt, off)
+    for s in range(nseg):
+        ref = uq[uo[s]:uo[s + 1]]
+        assert counts[s] == ref.size and got[off[s]:off[s] + ref.size].tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("dt", [PX_F32, PX_F64, PX_I64])
+def test_alone_equals_in_the_mix(gpu_ctx, dt):
+    a, off = seg_input(dt, 50 + dt, nseg=64)
+    d = DeviceBuffer.from_array(a)
+    out = DeviceBuffer(8 * (off.size - 1))
+    check(lib.pxb_seg_reduce(gpu_ctx.h, dt, d.ptr, u64p(off), off.size - 1, out.ptr), "seg_reduce")
+    o = DeviceBuffer(a.nbytes + 16)
+    check(lib.pxb_seg_inclusive_scan(gpu_ctx.h, dt, d.ptr, o.ptr, u64p(off), off.size - 1), "seg_scan")
+    gpu_ctx.sync()
+    mix = out.download(a.dtype, off.size - 1)
+    mixs = o.download(a.dtype, a.size)
+    one = DeviceBuffer(8)
+    for s in range(0, off.size - 1, 7):
+        so = np.array([off[s], off[s + 1]], dtype=np.uint64)
+        check(lib.pxb_seg_reduce(gpu_ctx.h, dt, d.ptr, u64p(so), 1, one.ptr), "seg_reduce alone")
+        gpu_ctx.sync()
+        assert one.download(a.dtype, 1).tobytes() == mix[s:s + 1].tobytes(), s
+        check(lib.pxb_seg_inclusive_scan(gpu_ctx.h, dt, d.ptr, o.ptr, u64p(so), 1), "seg_scan alone")
+        gpu_ctx.sync()
+        alone = o.download(a.dtype, a.size)[off[s]:off[s + 1]]
+        ref = mixs[off[s]:off[s + 1]]
+        if dt in FLOATS:  # identical tiling; the look-back may sum its window in another order
+            val, ab = pxo.scan_exact(a[off[s]:off[s + 1]])
+            assert np.all(np.abs(alone.astype(np.float64) - val) <= bound(dt, alone.size) * ab + 1e-300)
+            assert np.all(np.abs(ref.astype(np.float64) - val) <= bound(dt, ref.size) * ab + 1e-300)
+        else:
+            assert alone.tobytes() == ref.tobytes()
+
+
+def test_graph_plan_replays(gpu_ctx):
+    """A captured plan replayed several times, with other eager work between replays,
+    gives the eager result every time (no memset nodes; self-resetting scratch)."""
+    other = Context()
+    for op, dt in [(C.OP_REDUCE, PX_F64), (C.OP_SCAN, PX_U32), (C.OP_FLAGS, PX_I32), (C.OP_SORT, PX_U64)]:
+        a, off = seg_input(dt, 60 + op, nseg=120, hi=100_000)
+        nseg = off.size - 1
+        d = DeviceBuffer.from_array(a)
+        o = DeviceBuffer(max(a.nbytes, 8 * nseg) + 16)
+        cnt = DeviceBuffer(8 * nseg)
+        fl = make_desc(C.OP_FLAGS, dt, fn=C.PRED_LT, c0=300)
+        ctx = Context()
+        plan = lib.pxb_plan_create(ctx.h, op, dt, ctypes.byref(fl), d.ptr,
+                                   d.ptr if op == C.OP_SORT else o.ptr, u64p(off), nseg, cnt.ptr)
+        assert plan, "plan creation failed"
+        for rep in range(3):
+            if op == C.OP_SORT:
+                d.upload(a)
+            check(lib.pxb_plan_launch(plan), "plan launch")
+            ctx.sync()
+            # eager work on another context between replays
+            tmp = DeviceBuffer.from_array(np.arange(100_000, dtype=np.uint32))
+            tout = DeviceBuffer(8)
+            check(lib.pxd_reduce(other.h, PX_U32, tmp.ptr, 100_000, tout.ptr), "eager")
+            other.sync()
+            if op == C.OP_REDUCE:
+                got = o.download(a.dtype, nseg)
+                hi, ab = pxo.seg_sum_exact(a, off)
+                lens = np.diff(off.astype(np.int64))
+                assert all(abs(float(got[s]) - hi[s]) <= bound(dt, lens[s]) * ab[s] + 1e-300 for s in range(nseg))
+            elif op == C.OP_SCAN:
+                assert np.array_equal(o.download(a.dtype, a.size), pxo.seg_inclusive_scan(a, off))
+            elif op == C.OP_FLAGS:
+                counts = cnt.download(np.uint64, nseg)
+                got = o.download(a.dtype, a.size)
+                for s in range(nseg):
+                    ref = pxo.copy_if(fl, a[off[s]:off[s + 1]])
+                    assert counts[s] == ref.size and np.array_equal(got[off[s]:off[s] + ref.size], ref)
+            else:
+                assert np.array_equal(d.download(a.dtype, a.size), pxo.seg_sort(a, off))
+        lib.pxb_plan_destroy(plan)
+        ctx.close()
+    other.close()
+
+
+def make_server(budget, slots=4, max_batch=4096):
+    cfg = PxsConfig()
+    lib.pxs_config_default(ctypes.byref(cfg))
+    cfg.hbm_budget = budget
+    cfg.slots = slots
+    cfg.max_batch = max_batch
+    s = lib.pxs_create(ctypes.byref(cfg))
+    assert s, "pxs_create failed"
+    return s
+
+
+def put(server, a):
+    p = lib.pxs_alloc(server, max(a.nbytes, 1))
+    assert p
+    if a.nbytes:
+        check(lib.pxd_memcpy_h2d(p, a.ctypes.data, a.nbytes), "h2d")
+    return p
+
+
+def check_served(server, rid, op, dt, a, fl=None):
+    lib.pxs_wait(server, rid, -1.0)
+    out = ctypes.c_void_p()
+    cnt = ctypes.c_uint64()
+    lat = ctypes.c_double()
+    check(lib.pxs_result(server, rid, ctypes.byref(out), ctypes.byref(cnt), ctypes.byref(lat)), "result")
+    npd = NP_DTYPES[dt]
+    if op == C.OP_REDUCE:
+        got = np.empty(1, dtype=npd)
+        lib.pxd_memcpy_d2h(got.ctypes.data, out.value, got.nbytes)
+        if dt in FLOATS:
+            hi, lo, ab = pxo.sum_exact(a)
+            assert abs(float(got[0]) - (hi + lo)) <= bound(dt, a.size) * ab + 1e-300
+        else:
+            assert got[0] == pxo.reduce(a)
+    elif op == C.OP_SCAN:
+        got = np.empty(a.size, dtype=npd)
+        lib.pxd_memcpy_d2h(got.ctypes.data, out.value, got.nbytes)
+        if dt in FLOATS:
+            val, ab = pxo.scan_exact(a)
+            assert np.all(np.abs(got.astype(np.float64) - val) <= bound(dt, a.size) * ab + 1e-300)
+        else:
+            assert np.array_equal(got, pxo.inclusive_scan(a))
+    elif op == C.OP_SORT:
+        got = np.empty(a.size, dtype=npd)
+        lib.pxd_memcpy_d2h(got.ctypes.data, out.value, got.nbytes)
+        assert got.tobytes() == pxo.sort(a).tobytes()
+    else:
+        ref = pxo.copy_if(fl, a)
+        got = np.empty(ref.size, dtype=npd)
+        if ref.size:
+            lib.pxd_memcpy_d2h(got.ctypes.data, out.value, got.nbytes)
+        assert cnt.value == ref.size and got.tobytes() == ref.tobytes()
+    return lat.value
+
+
+def mixed_requests(seed, count, max_log=18):
+    rng = np.random.default_rng(seed)
+    ops = [C.OP_REDUCE, C.OP_SCAN, C.OP_SORT, C.OP_FLAGS]
+    dts = [PX_I32, PX_U64, PX_F32, PX_F64]
+    reqs = []
+    for i in range(count):
+        op = ops[rng.choice(4, p=[0.3, 0.3, 0.25, 0.15])]
+        dt = dts[rng.integers(0, 4)]
+        n = int(2 ** rng.uniform(4, max_log))
+        if dt in FLOATS:
+            a = rng.uniform(-1000, 1000, n).astype(NP_DTYPES[dt])
+        elif dt == PX_U64:
+            a = rng.integers(0, 1001, n).astype(np.uint64)
+        else:
+            a = rng.integers(-1000, 1001, n).astype(np.int32)
+        reqs.append((op, dt, a))
+    return reqs
+
+
+def submit(server, op, dt, a, p):
+    fl = make_desc(C.OP_FLAGS, dt, fn=C.PRED_LT, c0=500 if dt == PX_U64 else 0)
+    r = PxsRequest(op, dt, a.size, p, fl)
+    rid = lib.pxs_submit(server, ctypes.byref(r))
+    assert rid >= 0
+    return rid, fl
+
+
+def test_served_mix_matches_oracle_and_alone():
+    server = make_server(1 << 30)
+    reqs = mixed_requests(4, 120)
+    ptrs = [put(server, a) for _, _, a in reqs]
+    # all at once (mixed, batched, overlapping)
+    ids = [submit(server, op, dt, a, p) for (op, dt, a), p in zip(reqs, ptrs)]
+    lib.pxs_drain(server)
+    for (op, dt, a), (rid, fl) in zip(reqs, ids):
+        check_served(server, rid, op, dt, a, fl)
+        lib.pxs_release(server, rid)
+    st = PxsStats()
+    lib.pxs_get_stats(server, ctypes.byref(st))
+    assert st.completed == 120 and st.batches < 120  # requests were fused
+    # the same requests one at a time give the same results
+    for (op, dt, a), p in list(zip(reqs, ptrs))[::10]:
+        if op == C.OP_SORT:
+            lib.pxd_memcpy_h2d(p, a.ctypes.data, a.nbytes)
+        rid, fl = submit(server, op, dt, a, p)
+        check_served(server, rid, op, dt, a, fl)
+        lib.pxs_release(server, rid)
+    lib.pxs_destroy(server)
+
+
+def test_served_admission_waits_for_budget():
+    # budget: the inputs plus room for about three outputs at a time
+    n = 1 << 20
+    arrays = [np.arange(n, dtype=np.uint32) + i for i in range(12)]
+    server = make_server(12 * n * 4 + 3 * n * 4 + (1 << 20), slots=2)
+    ptrs = [put(server, a) for a in arrays]
+    ids = [submit(server, C.OP_SCAN, PX_U32, a, p)[0] for a, p in zip(arrays, ptrs)]
+    for a, rid in zip(arrays, ids):  # consume in order, freeing outputs as we go
+        check_served(server, rid, C.OP_SCAN, PX_U32, a)
+        lib.pxs_release(server, rid)
+    st = PxsStats()
+    lib.pxs_get_stats(server, ctypes.byref(st))
+    assert st.completed == 12 and st.pool_peak_bytes <= 12 * n * 4 + 3 * n * 4 + (1 << 20)
+    lib.pxs_destroy(server)
+
+
+def test_served_concurrent_submitters():
+    server = make_server(1 << 30)
+    reqs = mixed_requests(9, 80, max_log=16)
+    ptrs = [put(server, a) for _, _, a in reqs]
+    errors = []
+
+    def client(k):
+        try:
+            for i in range(k, len(reqs), 4):
+                op, dt, a = reqs[i]
+                rid, fl = submit(server, op, dt, a, ptrs[i])
+                check_served(server, rid, op, dt, a, fl)
+                lib.pxs_release(server, rid)
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    lib.pxs_destroy(server)

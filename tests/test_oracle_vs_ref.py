@@ -13,9 +13,18 @@ import pxo
 from parallax_amd import make_desc, consts as C
 from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_I64, PX_U32, PX_U64
 
-os.environ["PARALLAX_FORCE_CPU"] = "1"  # this process never uses the GPU path
 REF = pxo.load_ref()
 pytestmark = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference checkout)")
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _reference_lookups_miss():
+    """Every lookup of the reference's funnels misses while these tests run, so each call
+    takes the reference's host loop (the setting is read at each lookup and restored
+    afterwards, so it never leaks into GPU tests of the same session)."""
+    with pytest.MonkeyPatch.context() as mp:
+        mp.setenv("PARALLAX_FORCE_CPU", "1")
+        yield
 
 NAMES = {PX_I32: ("i32", ctypes.c_int32), PX_U32: ("u32", ctypes.c_uint32),
          PX_I64: ("i64", ctypes.c_int64), PX_U64: ("u64", ctypes.c_uint64),
