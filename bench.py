@@ -190,6 +190,222 @@ def verify(X, D, su, sf, kept):
     return out
 
 
+# ------------------------------------------------------------------ batched forms
+def batched(args):
+    """B1: 16,384 f32 segments, lengths log-uniform [1, 2^18], seed 2: segmented reduce and
+    inclusive scan, each one persistent launch replayed from a hipGraph.
+    B2: 4,096 u64 segments, lengths log-uniform [16, 2^18], keys uniform [0, 2^20), seed 3:
+    segmented sort then segmented unique, replayed from hipGraphs.  value = algorithmic
+    bytes of B1 + B2 / device time of one replay of the four graphs."""
+    import math
+    from parallax_amd import lib, check, make_desc, consts as C, datagen
+    from parallax_amd.device import Context, DeviceBuffer
+    from parallax_amd.dtypes import PX_F32, PX_U64
+
+    s = args.scale
+    nseg1 = max(1, int(16384 * s))
+    lens1 = datagen.log_uniform_lengths(2, nseg1, 1, 1 << 18, stream=1)
+    off1 = datagen.offsets_from_lengths(lens1)
+    x1 = datagen.normal_f32(2, int(off1[-1]), stream=2)
+    nseg2 = max(1, int(4096 * s))
+    lens2 = datagen.log_uniform_lengths(3, nseg2, 16, 1 << 18, stream=1)
+    off2 = datagen.offsets_from_lengths(lens2)
+    x2 = datagen.int_range(3, int(off2[-1]), 0, 1 << 20, np.uint64, stream=2)
+    log(f"[bench] batched inputs: B1 {off1[-1]} f32 in {nseg1} segments, B2 {off2[-1]} u64 in {nseg2}")
+    ctx = Context()
+    d1 = DeviceBuffer.from_array(x1)
+    r1 = DeviceBuffer(8 * nseg1)
+    sc1 = DeviceBuffer(x1.nbytes + 16)
+    d2 = DeviceBuffer.from_array(x2)
+    u2 = DeviceBuffer(x2.nbytes + 16)
+    c2 = DeviceBuffer(8 * nseg2)
+    o1 = off1.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    o2 = off2.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    ud = make_desc(C.OP_FLAGS, PX_U64, fn=C.PRED_UNIQUE)
+    plans = {
+        "seg_reduce_f32": lib.pxb_plan_create(ctx.h, C.OP_REDUCE, PX_F32, None, d1.ptr, r1.ptr, o1, nseg1, None),
+        "seg_scan_f32": lib.pxb_plan_create(ctx.h, C.OP_SCAN, PX_F32, None, d1.ptr, sc1.ptr, o1, nseg1, None),
+        "seg_sort_u64": lib.pxb_plan_create(ctx.h, C.OP_SORT, PX_U64, None, d2.ptr, d2.ptr, o2, nseg2, None),
+        "seg_unique_u64": lib.pxb_plan_create(ctx.h, C.OP_FLAGS, PX_U64, ctypes.byref(ud), d2.ptr, u2.ptr, o2, nseg2, c2.ptr),
+    }
+    assert all(plans.values()), plans
+    # parity (after the plan warm-up runs): per segment vs float64 reference / numpy
+    ctx.sync()
+    got = r1.download(np.float32, nseg1)
+    idx = off1[:-1].astype(np.int64)
+    exact = np.add.reduceat(x1.astype(np.float64), idx) if x1.size else np.zeros(nseg1)
+    absum = np.add.reduceat(np.abs(x1.astype(np.float64)), idx)
+    exact[lens1 == 0] = 0
+    absum[lens1 == 0] = 0
+    eps = float(np.finfo(np.float32).eps)
+    bnd = np.array([2 * max(1, math.ceil(math.log2(max(int(l), 2)))) * eps for l in lens1]) * absum
+    ratio = np.abs(got.astype(np.float64) - exact) / np.maximum(bnd, 1e-300)
+    assert np.all(ratio <= 1.0), float(ratio.max())
+    # sort + unique (the unique plan ran on the pre-sort data during warm-up: re-run in order)
+    d2.upload(x2)
+    check(lib.pxb_plan_launch(plans["seg_sort_u64"]), "sort replay")
+    check(lib.pxb_plan_launch(plans["seg_unique_u64"]), "unique replay")
+    ctx.sync()
+    srt = d2.download(np.uint64, x2.size)
+    cnt = c2.download(np.uint64, nseg2)
+    uq = u2.download(np.uint64, x2.size)
+    kept_total = int(cnt.sum())
+    for i in range(0, nseg2, max(1, nseg2 // 64)):
+        seg = np.sort(x2[off2[i]:off2[i + 1]])
+        assert np.array_equal(srt[off2[i]:off2[i + 1]], seg)
+        u = np.unique(seg)
+        assert cnt[i] == u.size and np.array_equal(uq[off2[i]:off2[i] + u.size], u)
+    parity = {"seg_reduce_f32_max_err_over_bound": float(ratio.max()),
+              "sort_unique_u64": "bit-exact (sampled 64 segments)"}
+    # timed replays
+    times = {k: [] for k in plans}
+    for it in range(args.warmup + args.steps):
+        d2.upload(x2)  # fresh unsorted keys for the sort replay (outside the timed graphs)
+        for k in ["seg_reduce_f32", "seg_scan_f32", "seg_sort_u64", "seg_unique_u64"]:
+            ctx.timer_begin()
+            check(lib.pxb_plan_launch(plans[k]), k)
+            ms = ctx.timer_end()
+            if it >= args.warmup:
+                times[k].append(ms)
+    n1, n2 = int(off1[-1]), int(off2[-1])
+    alg = {"seg_reduce_f32": n1 * 4 + (nseg1 + 1) * 8, "seg_scan_f32": 2 * n1 * 4 + (nseg1 + 1) * 8,
+           "seg_sort_u64": 2 * n2 * 8 + (nseg2 + 1) * 8, "seg_unique_u64": n2 * 8 + kept_total * 8 + (nseg2 + 1) * 8}
+    per = {}
+    total_ms = 0.0
+    for k in plans:
+        ms = float(np.median(times[k]))
+        total_ms += ms
+        gbps = alg[k] / (ms * 1e-3) / 1e9
+        per[k] = {"ms": round(ms, 4), "alg_bytes": alg[k], "gbps": round(gbps, 1),
+                  "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 1)}
+    for p in plans.values():
+        lib.pxb_plan_destroy(p)
+    value = sum(alg.values()) / (total_ms * 1e-3) / 1e9
+    dom = max(per, key=lambda k: per[k]["ms"])
+    return {"value": round(value, 1), "unit": "GB/s", "ms_per_step": round(total_ms, 3),
+            "dtype": "mixed(f32,u64)",
+            "config": {"workload": f"batched: B1 {nseg1} f32 segments log-uniform [1,2^18] (seg reduce + "
+                                   f"inclusive scan), B2 {nseg2} u64 segments log-uniform [16,2^18] "
+                                   "(seg sort + unique); hipGraph replays", "scale": s},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": per[dom]["gbps"], "peak": HBM_PEAK_GBPS,
+                         "unit": "GB/s", "frac": round(per[dom]["gbps"] / HBM_PEAK_GBPS, 4), "traffic": None},
+            "per_primitive": per, "parity": parity}
+
+
+# ------------------------------------------------------------------ served form
+def served(args):
+    """Served mixed stream (seed 4): requests of 30% reduce, 30% scan, 25% sort, 15%
+    copy_if; sizes log-uniform [2^10, 2^24]; dtypes uniform over {i32, u64, f32, f64};
+    values uniform in [-1000, 1000] ([0, 1000] unsigned); 64 GB budget; inputs resident in
+    the server's pool.  Reported: saturated req/s (all requests queued at once), p50/p99
+    latency under Poisson arrivals at half that rate, and the same requests issued as
+    plain per-call synchronous calls (device-resident operands) for the speed-up."""
+    import threading
+    from parallax_amd import lib, check, make_desc, consts as C, datagen
+    from parallax_amd._lib import PxsConfig, PxsRequest, PxsStats
+    from parallax_amd.device import Context
+    from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_U64
+
+    nreq = max(8, int(400 * args.scale))
+    rng_u = datagen.uniform01_f64(4, 4 * nreq, stream=1)
+    ops = np.array([C.OP_REDUCE, C.OP_SCAN, C.OP_SORT, C.OP_FLAGS])
+    op_of = ops[np.searchsorted(np.cumsum([0.3, 0.3, 0.25, 0.15]), rng_u[:nreq], side="right").clip(0, 3)]
+    dts = [PX_I32, PX_U64, PX_F32, PX_F64]
+    dt_of = [dts[int(u * 4) % 4] for u in rng_u[nreq:2 * nreq]]
+    sizes = np.floor(2.0 ** (10 + 14 * rng_u[2 * nreq:3 * nreq])).astype(np.int64)
+    cfg = PxsConfig()
+    lib.pxs_config_default(ctypes.byref(cfg))
+    server = lib.pxs_create(ctypes.byref(cfg))
+    assert server, "pxs_create failed"
+    reqs = []
+    for i in range(nreq):
+        dt, n = dt_of[i], int(sizes[i])
+        if dt == PX_U64:
+            a = datagen.int_range(4, n, 0, 1001, np.uint64, stream=100 + i)
+        elif dt == PX_I32:
+            a = datagen.int_range(4, n, -1000, 1001, np.int32, stream=100 + i)
+        else:
+            a = (datagen.uniform01_f64(4, n, stream=100 + i) * 2000.0 - 1000.0).astype(NP_DTYPES[dt])
+        p = lib.pxs_alloc(server, a.nbytes)
+        assert p
+        check(lib.pxd_memcpy_h2d(p, a.ctypes.data, a.nbytes), "upload")
+        fl = make_desc(C.OP_FLAGS, dt, fn=C.PRED_LT, c0=500 if dt == PX_U64 else 0)
+        reqs.append((int(op_of[i]), dt, a, p, fl))
+    total_bytes = sum(a.nbytes * (1 if op == C.OP_REDUCE else 2) for op, _, a, _, _ in reqs)
+    log(f"[bench] served: {nreq} requests, {total_bytes / 1e9:.2f} GB algorithmic")
+
+    def restore_sorts():
+        for op, dt, a, p, _ in reqs:
+            if op == C.OP_SORT:
+                check(lib.pxd_memcpy_h2d(p, a.ctypes.data, a.nbytes), "restore")
+
+    def run_served(arrivals):
+        lib.pxs_reset_stats(server)
+        ids = []
+        t0 = time.perf_counter()
+        for (op, dt, a, p, fl), t_arr in zip(reqs, arrivals):
+            d = t0 + t_arr - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            rid = lib.pxs_submit(server, ctypes.byref(PxsRequest(op, dt, a.size, p, fl)))
+            assert rid >= 0
+            ids.append(rid)
+        lib.pxs_drain(server)
+        st = PxsStats()
+        lib.pxs_get_stats(server, ctypes.byref(st))
+        for rid in ids:
+            lib.pxs_release(server, rid)
+        return st
+
+    # warm-up then saturated throughput
+    restore_sorts()
+    run_served([0.0] * nreq)
+    restore_sorts()
+    sat = run_served([0.0] * nreq)
+    # Poisson arrivals at half the saturated rate
+    lam = 0.5 * sat.req_per_s
+    gaps = -np.log1p(-datagen.uniform01_f64(4, nreq, stream=9)) / lam
+    restore_sorts()
+    poi = run_served(np.cumsum(gaps) - gaps[0])
+    # plain per-call: the same requests, one synchronous call after another
+    ctx = Context()
+    out = lib.pxd_malloc(max(a.nbytes for _, _, a, _, _ in reqs) + 64)
+    cnt = lib.pxd_malloc(64)
+    restore_sorts()
+
+    def plain_all():
+        for op, dt, a, p, fl in reqs:
+            if op == C.OP_REDUCE:
+                check(lib.pxd_reduce(ctx.h, dt, p, a.size, out), "reduce")
+            elif op == C.OP_SCAN:
+                check(lib.pxd_inclusive_scan(ctx.h, dt, p, out, a.size), "scan")
+            elif op == C.OP_SORT:
+                check(lib.pxd_sort(ctx.h, dt, p, a.size), "sort")
+            else:
+                check(lib.pxd_copy_if(ctx.h, ctypes.byref(fl), p, out, a.size, cnt), "copy_if")
+            ctx.sync()
+    plain_all()
+    restore_sorts()
+    t0 = time.perf_counter()
+    plain_all()
+    plain_s = time.perf_counter() - t0
+    plain_rps = nreq / plain_s
+    lib.pxd_free(out)
+    lib.pxd_free(cnt)
+    lib.pxs_destroy(server)
+    return {"value": round(sat.req_per_s, 1), "unit": "req/s", "ms_per_step": round(1e3 / sat.req_per_s, 4),
+            "dtype": "mixed(i32,u64,f32,f64)",
+            "config": {"workload": f"served mixed stream: {nreq} requests (30% reduce, 30% scan, 25% sort, "
+                                   "15% copy_if), sizes log-uniform [2^10,2^24], dtypes {i32,u64,f32,f64}, "
+                                   "seed 4, 64 GB pool, 4 slots", "scale": args.scale},
+            "req_per_s": {"served_saturated": round(sat.req_per_s, 1), "served_poisson": round(poi.req_per_s, 1),
+                          "plain_per_call": round(plain_rps, 1)},
+            "latency_ms": {"poisson_rate_req_s": round(lam, 1), "p50": round(poi.p50_ms, 3), "p99": round(poi.p99_ms, 3),
+                           "saturated_p50": round(sat.p50_ms, 3), "saturated_p99": round(sat.p99_ms, 3)},
+            "speedup_vs_plain": round(sat.req_per_s / plain_rps, 2),
+            "roofline": None}
+
+
 # ------------------------------------------------------------------ CPU baseline
 CPU_SAMPLE = {"n_stream": 1 << 26, "n_sort_u32": 1 << 23, "n_sort_f64": 1 << 22}
 
@@ -303,10 +519,11 @@ def main():
         sys.exit(2)
     if args.workload == "plain":
         r = plain_suite(args)
+    elif args.workload == "batched":
+        r = batched(args)
     else:
-        print(f"bench.py: workload {args.workload} is not available yet", file=sys.stderr)
-        sys.exit(2)
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
+        r = served(args)
+    cpu = None if (args.no_cpu_baseline or args.workload != "plain") else cpu_baseline()
     line = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

@@ -54,7 +54,7 @@ def _load():
         ("pxo_map", None, [dp, vp, sz, vp, sz]),
         ("pxo_transform", None, [dp, vp, vp, sz, vp, sz]),
         ("pxo_seg_reduce", None, [i, vp, u64p, sz, vp]),
-        ("pxo_seg_sum_exact", None, [i, vp, u64p, sz, dbl, dbl]),
+        ("pxo_seg_sum_exact", None, [i, vp, u64p, sz, vp, vp]),
         ("pxo_seg_inclusive_scan", None, [i, vp, vp, u64p, sz]),
         ("pxo_seg_sort", None, [i, vp, u64p, sz]),
         ("pxo_seg_unique", sz, [i, vp, vp, u64p, sz, u64p]),

@@ -191,7 +191,7 @@ int zero_dev(StreamContext& c, void* p, size_t bytes) {
   return dev_err(pxk::fill_u32(static_cast<uint32_t*>(p), bytes / 4, 0u, c.stream()), "fill");
 }
 
-pxk::SegGeom plain_geom(uint64_t n, int dtype, int k, const void* in) {
+pxk::SegGeom plain_geom(uint64_t n, int dtype, pxk::TileCfg k, const void* in) {
   pxk::SegGeom g{};
   const uint32_t tile = pxk::tile_elems(dtype, k);
   const int64_t shift = pxk::elem_shift(in, pxk::dtype_size(dtype));
@@ -211,7 +211,7 @@ bool valid_spec(const SegSpec& sp) {
 }
 
 // Upload segment bounds + tile tables as one blob and fill a segmented geometry.
-int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, int k, const void* in,
+int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, pxk::TileCfg k, const void* in,
              pxk::SegGeom& g, TileTables& tt) {
   const uint32_t nseg = sp.nseg;
   const uint32_t tile = pxk::tile_elems(dtype, k);
@@ -265,7 +265,7 @@ int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n,
   if (n == 0) return PX_OK;
   ++g_stats.scan;
   ++g_stats.launches;
-  const pxk::SegGeom g = plain_geom(n, dtype, pxk::kScanK, in);
+  const pxk::SegGeom g = plain_geom(n, dtype, pxk::tune().scan, in);
   if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0, 0))
     return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
@@ -286,7 +286,7 @@ int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* ou
   ++g_stats.compact;
   ++g_stats.launches;
   if (n == 0) return zero_dev(c, count_dev, sizeof(uint64_t));
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::kScanK, in);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().scan, in);
   const size_t words = n >= (1ull << 32) ? 2 : 1;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
@@ -304,7 +304,7 @@ int run_partition(StreamContext& c, const px_desc& flags, const void* in, void* 
   if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
   int rc = dev_err(pxk::count_if(flags, in, n, num_true_dev, c.scratch(), c.stream()), "count");
   if (rc) return rc;
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::kScanK, in);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().scan, in);
   const size_t words = n >= (1ull << 32) ? 2 : 1;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
@@ -361,7 +361,7 @@ int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& s
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, dtype, pxk::kSegReduceK, in, g, tt);
+  int rc = seg_geom(c, sp, dtype, pxk::tune().seg_reduce, in, g, tt);
   if (rc) return rc;
   if (!c.reserve(0, static_cast<size_t>(g.ntiles) * 16, sp.nseg)) return PX_ERR_NOMEM;
   return dev_err(pxk::seg_reduce(dtype, in, g, out_dev, c.scratch(), c.stream()), "seg_reduce");
@@ -374,7 +374,7 @@ int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const S
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, dtype, pxk::kScanK, in, g, tt);
+  int rc = seg_geom(c, sp, dtype, pxk::tune().scan, in, g, tt);
   if (rc) return rc;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0, 0))
     return PX_ERR_NOMEM;
@@ -390,7 +390,7 @@ int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, flags.dtype, pxk::kScanK, in, g, tt);
+  int rc = seg_geom(c, sp, flags.dtype, pxk::tune().scan, in, g, tt);
   if (rc) return rc;
   if (!c.reserve(static_cast<size_t>(g.ntiles), 0, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;

@@ -22,20 +22,21 @@ namespace pxk {
 
 namespace {
 
-template <class T, int K, class C, int MODE, bool UNIQUE>
-__global__ __launch_bounds__(256) void k_compact(const T* in_al, int64_t in_shift, T* out,
+template <class T, int BLOCK, int K, class C, int MODE, bool UNIQUE>
+__global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_shift, T* out,
                                                  SegGeom g, px_desc pd, uint64_t* counts,
                                                  const uint64_t* num_true_dev, uint64_t* status,
                                                  Ctl* ctl) {
   constexpr int VEC = 16 / sizeof(T);
-  constexpr int TILE = 256 * K * VEC;
+  constexpr int TILE = BLOCK * K * VEC;
+  constexpr int NW = BLOCK / kWave;
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
 
   __shared__ T s_stage[TILE];
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
-  __shared__ C s_w[4];
+  __shared__ C s_w[NW];
   __shared__ C s_prefix;
 
   const Pred<T> pred(pd);
@@ -101,10 +102,10 @@ __global__ __launch_bounds__(256) void k_compact(const T* in_al, int64_t in_shif
     if (lane == 0) s_w[wid] = run;
     __syncthreads();
     C wave_off = 0;
-    if (wid >= 1) wave_off = s_w[0];
-    if (wid >= 2) wave_off += s_w[1];
-    if (wid >= 3) wave_off += s_w[2];
-    const C tile_kept = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    for (int j = 0; j < wid; ++j) wave_off += s_w[j];
+    C tile_kept = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) tile_kept += s_w[j];
     const int64_t vstart = ti.w0 > ti.lo ? ti.w0 : ti.lo;  // first valid position of the tile
     const int64_t vend = (ti.w0 + TILE) < ti.hi ? (ti.w0 + TILE) : ti.hi;
     const C tile_valid = static_cast<C>(vend > vstart ? vend - vstart : 0);
@@ -156,12 +157,12 @@ __global__ __launch_bounds__(256) void k_compact(const T* in_al, int64_t in_shif
     const uint64_t P = s_prefix;
     const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
     T* o = out + seg_out + P;
-    for (uint32_t j = threadIdx.x; j < tile_kept; j += 256) o[j] = s_stage[j];
+    for (uint32_t j = threadIdx.x; j < tile_kept; j += BLOCK) o[j] = s_stage[j];
     if (MODE == PX_SCATTER_PARTITION) {
       const uint64_t num_true = *num_true_dev;
       const uint64_t rej_before = static_cast<uint64_t>(vstart - ti.lo) - P;
       T* orj = out + seg_out + num_true + rej_before;
-      for (uint32_t j = tile_kept + threadIdx.x; j < tile_valid; j += 256)
+      for (uint32_t j = tile_kept + threadIdx.x; j < tile_valid; j += BLOCK)
         orj[j - tile_kept] = s_stage[j];
     }
     __syncthreads();
@@ -205,11 +206,28 @@ __global__ __launch_bounds__(256) void k_count(const T* in, uint64_t n, px_desc 
 }
 
 template <class T, class C, int MODE, bool UNIQUE>
-void launch_compact_t(uint32_t grid, hipStream_t stream, const T* in_al, int64_t in_shift, T* out,
-                      const SegGeom& g, const px_desc& pd, uint64_t* counts,
-                      const uint64_t* num_true_dev, const Scratch& s) {
-  k_compact<T, kScanK, C, MODE, UNIQUE><<<grid, 256, 0, stream>>>(
-      in_al, in_shift, out, g, pd, counts, num_true_dev, s.status, s.ctl);
+hipError_t launch_compact_t(hipStream_t stream, const T* in_al, int64_t in_shift, T* out,
+                            const SegGeom& g, const px_desc& pd, uint64_t* counts,
+                            const uint64_t* num_true_dev, const Scratch& s) {
+  const TileCfg c = tune().scan;
+  uint32_t grid = g.ntiles;
+  if (tune().scan_bpc > 0) {
+    const uint32_t cap = s.max_blocks / 8 * static_cast<uint32_t>(tune().scan_bpc);
+    if (grid > cap) grid = cap;
+  }
+  if (grid == 0) return hipSuccess;
+#define PXK_CMP(B, KK)                                                                        \
+  if (c.block == B && c.k == KK) {                                                            \
+    k_compact<T, B, KK, C, MODE, UNIQUE><<<grid, B, 0, stream>>>(                             \
+        in_al, in_shift, out, g, pd, counts, num_true_dev, s.status, s.ctl);                  \
+    return hipGetLastError();                                                                 \
+  }
+  PXK_CMP(256, 4)
+  PXK_CMP(256, 8)
+  PXK_CMP(512, 4)
+  PXK_CMP(512, 8)
+#undef PXK_CMP
+  return hipErrorInvalidValue;
 }
 
 template <class T>
@@ -218,11 +236,9 @@ hipError_t compact_t(const px_desc& pd, int mode, const void* in, void* out, con
                      hipStream_t stream) {
   const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
   const int64_t in_shift = elem_shift(in, sizeof(T));
-  const uint32_t grid = g.ntiles < s.max_blocks ? g.ntiles : s.max_blocks;
-  if (grid == 0) return hipSuccess;
-  // total element count decides the count width
-  uint64_t total = g.n;
-  const bool wide = (g.seg_lo != nullptr) ? false : (total >= (1ull << 32));
+  if (g.ntiles == 0) return hipSuccess;
+  // the total element count decides the count width (a segment stays below 2^32)
+  const bool wide = (g.seg_lo != nullptr) ? false : (g.n >= (1ull << 32));
   const size_t words = wide ? 2 : 1;
   if (static_cast<size_t>(g.ntiles) * words > s.status_words) return hipErrorInvalidValue;
   const bool uniq = (pd.fn == PX_PRED_UNIQUE);
@@ -230,21 +246,17 @@ hipError_t compact_t(const px_desc& pd, int mode, const void* in, void* out, con
   if (mode == PX_SCATTER_PARTITION) {
     if (g.seg_lo != nullptr || num_true_dev == nullptr) return hipErrorInvalidValue;
     if (wide)
-      launch_compact_t<T, uint64_t, PX_SCATTER_PARTITION, false>(grid, stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
-    else
-      launch_compact_t<T, uint32_t, PX_SCATTER_PARTITION, false>(grid, stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
-  } else if (uniq) {
-    if (wide)
-      launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, true>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
-    else
-      launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, true>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
-  } else {
-    if (wide)
-      launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, false>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
-    else
-      launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, false>(grid, stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+      return launch_compact_t<T, uint64_t, PX_SCATTER_PARTITION, false>(stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
+    return launch_compact_t<T, uint32_t, PX_SCATTER_PARTITION, false>(stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
   }
-  return hipGetLastError();
+  if (uniq) {
+    if (wide)
+      return launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, true>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+    return launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, true>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+  }
+  if (wide)
+    return launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, false>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+  return launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, false>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
 }
 
 template <class T>

@@ -21,7 +21,22 @@ namespace pxk {
 
 namespace {
 
-template <class T, int VEC>
+using u32x4 = __attribute__((ext_vector_type(4))) unsigned int;
+
+// 16-byte load, optionally non-temporal (streamed data is read once: keep it out of the
+// caches the look-back and partials use).
+template <class V, bool NT>
+__device__ __forceinline__ V load16(const V* p) {
+  static_assert(sizeof(V) == 16, "16-byte vectors only");
+  if constexpr (NT) {
+    const u32x4 r = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return __builtin_bit_cast(V, r);
+  } else {
+    return *p;
+  }
+}
+
+template <class T, int VEC, int UNROLL, bool NT>
 __global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int64_t lo,
                                                 int64_t hi,
                                                 typename Traits<T>::A* __restrict__ partials,
@@ -37,25 +52,26 @@ __global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int
   const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
   int64_t i = a + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   A acc = acc_zero<A>();
-  for (; i + 3 * stride < b; i += 4 * stride) {
-    const V v0 = vin[i];
-    const V v1 = vin[i + stride];
-    const V v2 = vin[i + 2 * stride];
-    const V v3 = vin[i + 3 * stride];
+  A acc2 = acc_zero<A>();  // two chains for more add-level parallelism
+  for (; i + (UNROLL - 1) * stride < b; i += UNROLL * stride) {
+    V v[UNROLL];
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v0.v[j]));
+    for (int u = 0; u < UNROLL; ++u) v[u] = load16<V, NT>(vin + i + u * stride);
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v1.v[j]));
+    for (int u = 0; u < UNROLL; ++u) {
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v2.v[j]));
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v3.v[j]));
+      for (int j = 0; j < VEC; ++j) {
+        if (u & 1) acc2 = acc_add(acc2, to_acc(v[u].v[j]));
+        else acc = acc_add(acc, to_acc(v[u].v[j]));
+      }
+    }
   }
   for (; i < b; i += stride) {
-    const V v0 = vin[i];
+    const V v0 = load16<V, NT>(vin + i);
 #pragma unroll
     for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v0.v[j]));
   }
+  acc = acc_add(acc, acc2);
   if (blockIdx.x == 0 && threadIdx.x < 2 * VEC) {
     // head [lo, min(a*VEC, hi)) and tail [max(a,b)*VEC, hi): fewer than VEC each
     int64_t pos;
@@ -175,13 +191,24 @@ hipError_t reduce_t(const void* in, uint64_t n, void* out_dev, const Scratch& s,
   const int64_t lo = elem_shift(in, sizeof(T));
   const int64_t hi = lo + static_cast<int64_t>(n);
   const int64_t nvec = hi / VEC - (lo + VEC - 1) / VEC;
-  int64_t grid = (nvec + 1023) / 1024;
+  const Tune& t = tune();
+  const int unroll = t.red_unroll == 4 ? 4 : 8;
+  int64_t grid = (nvec + 256 * unroll - 1) / (256 * unroll);
   if (grid < 1) grid = 1;
-  const int64_t cap = static_cast<int64_t>(s.max_blocks);
+  const int64_t cap = static_cast<int64_t>(s.max_blocks / 8) * (t.red_bpc > 0 ? t.red_bpc : 8);
   if (grid > cap) grid = cap;
   if (static_cast<size_t>(grid) * sizeof(A) > s.partial_bytes) return hipErrorInvalidValue;
-  k_reduce<T, VEC><<<static_cast<uint32_t>(grid), 256, 0, stream>>>(
-      in_al, lo, hi, reinterpret_cast<A*>(s.partials), s.ctl, reinterpret_cast<T*>(out_dev));
+  const uint32_t gb = static_cast<uint32_t>(grid);
+  A* part = reinterpret_cast<A*>(s.partials);
+  T* o = reinterpret_cast<T*>(out_dev);
+  if (unroll == 4 && t.red_nt)
+    k_reduce<T, VEC, 4, true><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
+  else if (unroll == 4)
+    k_reduce<T, VEC, 4, false><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
+  else if (t.red_nt)
+    k_reduce<T, VEC, 8, true><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
+  else
+    k_reduce<T, VEC, 8, false><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
   return hipGetLastError();
 }
 

@@ -23,21 +23,22 @@ namespace pxk {
 
 namespace {
 
-template <class T, int K, bool EXCL>
-__global__ __launch_bounds__(256) void k_scan(const T* in_al, int64_t in_shift, T* out,
+template <class T, int BLOCK, int K, bool EXCL>
+__global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift, T* out,
                                               int64_t out_shift, SegGeom g,
                                               typename Traits<T>::A init, uint64_t* status,
                                               Ctl* ctl) {
   using A = typename Traits<T>::A;
   constexpr int VEC = 16 / sizeof(T);
-  constexpr int TILE = 256 * K * VEC;
+  constexpr int TILE = BLOCK * K * VEC;
+  constexpr int NW = BLOCK / kWave;
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
   V* vout = reinterpret_cast<V*>(reinterpret_cast<uintptr_t>(out) & ~uintptr_t(15));
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
-  __shared__ A s_w[4];
+  __shared__ A s_w[NW];
   __shared__ A s_prefix;
 
   const int lane = lane_id();
@@ -90,12 +91,12 @@ __global__ __launch_bounds__(256) void k_scan(const T* in_al, int64_t in_shift, 
 
     // ---- wave offsets and tile total in a fixed order
     A wave_off = acc_zero<A>();
-    if (wid >= 1) wave_off = s_w[0];
-    if (wid >= 2) wave_off = acc_add(wave_off, s_w[1]);
-    if (wid >= 3) wave_off = acc_add(wave_off, s_w[2]);
+    for (int j = 0; j < wid; ++j) wave_off = acc_add(wave_off, s_w[j]);
 
     if (wid == 0) {
-      const A tile_tot = acc_add(acc_add(acc_add(s_w[0], s_w[1]), s_w[2]), s_w[3]);
+      A tile_tot = s_w[0];
+#pragma unroll
+      for (int j = 1; j < NW; ++j) tile_tot = acc_add(tile_tot, s_w[j]);
       A prefix;
       if (tile == ti.first_tile) {
         prefix = EXCL ? init : acc_zero<A>();
@@ -162,15 +163,28 @@ hipError_t scan_t(const void* in, void* out, const SegGeom& g, bool exclusive, c
       iv = static_cast<A>(t);
     }
   }
-  const uint32_t grid = g.ntiles < s.max_blocks ? g.ntiles : s.max_blocks;
+  const TileCfg c = tune().scan;
+  uint32_t grid = g.ntiles;
+  if (tune().scan_bpc > 0) {
+    const uint32_t cap = s.max_blocks / 8 * static_cast<uint32_t>(tune().scan_bpc);
+    if (grid > cap) grid = cap;
+  }
   if (grid == 0) return hipSuccess;
-  if (exclusive)
-    k_scan<T, kScanK, true><<<grid, 256, 0, stream>>>(in_al, in_shift, reinterpret_cast<T*>(out),
-                                                       out_shift, g, iv, s.status, s.ctl);
-  else
-    k_scan<T, kScanK, false><<<grid, 256, 0, stream>>>(in_al, in_shift, reinterpret_cast<T*>(out),
-                                                        out_shift, g, iv, s.status, s.ctl);
-  return hipGetLastError();
+  T* o = reinterpret_cast<T*>(out);
+#define PXK_SCAN(B, KK)                                                                            \
+  if (c.block == B && c.k == KK) {                                                                 \
+    if (exclusive)                                                                                 \
+      k_scan<T, B, KK, true><<<grid, B, 0, stream>>>(in_al, in_shift, o, out_shift, g, iv, s.status, s.ctl); \
+    else                                                                                           \
+      k_scan<T, B, KK, false><<<grid, B, 0, stream>>>(in_al, in_shift, o, out_shift, g, iv, s.status, s.ctl); \
+    return hipGetLastError();                                                                      \
+  }
+  PXK_SCAN(256, 4)
+  PXK_SCAN(256, 8)
+  PXK_SCAN(512, 4)
+  PXK_SCAN(512, 8)
+#undef PXK_SCAN
+  return hipErrorInvalidValue;
 }
 
 }  // namespace

@@ -32,10 +32,25 @@ constexpr uint32_t kEpochMax = (1u << 30) - 1;
 
 struct SegGeom;
 
-// Tile sizes (items per thread = K * 16 / sizeof(T)); the autotuner may pick others
-// for the segmented forms through the *_k variants.
+// Launch shapes.  A tile of the tiled kernels is `block` threads x `k` 16-byte vectors
+// per thread.  Defaults can be overridden for sweeps with PARALLAX_TUNE, e.g.
+// "scan_block=512,scan_k=8,scan_bpc=2,red_unroll=8,red_bpc=4,red_nt=1"; only the
+// compiled combinations below are accepted (others keep the default).
+struct TileCfg {
+  int block;
+  int k;
+};
+struct Tune {
+  TileCfg scan{512, 8};       // scan and compaction (look-back kernels)
+  int scan_bpc = 2;           // resident blocks per CU (0: one tile per block, grid = tiles)
+  TileCfg seg_reduce{256, 4};
+  int red_unroll = 8;         // 16-byte loads in flight per thread in the plain reduce
+  int red_bpc = 4;            // blocks per CU of the plain reduce
+  int red_nt = 1;             // non-temporal loads in the plain reduce
+};
+Tune& tune();
+bool tile_cfg_supported(TileCfg c);
 constexpr int kSegReduceK = 4;
-constexpr int kScanK = 4;
 
 // Scratch of one stream context.  All of it is device memory; ctl is zeroed once with
 // epoch = 1 and is left clean by every kernel.
@@ -77,9 +92,9 @@ inline size_t carry_bytes(int dtype) {
     default: return 0;
   }
 }
-// Elements per tile of the tiled kernels for a given K.
-inline uint32_t tile_elems(int dtype, int k) {
-  return static_cast<uint32_t>(256 * k * (16 / dtype_size(dtype)));
+// Elements per tile of the tiled kernels.
+inline uint32_t tile_elems(int dtype, TileCfg c) {
+  return static_cast<uint32_t>(c.block * c.k * (16 / dtype_size(dtype)));
 }
 
 // ---- reduce (k_reduce.hip)

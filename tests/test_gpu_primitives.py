@@ -227,8 +227,8 @@ MAPS = [(C.MAP_IDENTITY, 0, 0), (C.MAP_FILL, 9, 0), (C.MAP_ADD, 3, 0), (C.MAP_MU
 @pytest.mark.parametrize("mp", MAPS)
 def test_transform_same_type(gpu_ctx, dt, mp):
     fn, c0, c1 = mp
-    if dt in (PX_U32, PX_U64) and c0 < 0:
-        c0 = 2
+    if dt in (PX_U32, PX_U64):
+        c0, c1 = abs(c0), abs(c1)
     a = gen(dt, 10007, 900 + dt, *((None, None) if dt in FLOATS else (-1000, 1000) if dt in (PX_I32, PX_I64) else (0, 1000)))
     d = make_desc(C.OP_TRANSFORM, dt, aux=dt, fn=fn, c0=c0, c1=c1)
     ib, ip = upload(a)
