@@ -56,8 +56,6 @@ __device__ __forceinline__ void cmpx(T& a, T& b) {
   }
 }
 
-constexpr int kLocalThreads = 512;
-
 // Segment of the current block: plain (one segment) or from the block table.
 struct SortSeg {
   uint64_t base, n;
@@ -87,58 +85,166 @@ __device__ __forceinline__ SortSeg sort_seg(const uint64_t* seg_off, const uint3
   return s;
 }
 
-// LDS kernel.  MERGE == false: full sort of each tile (levels 1 .. tile).  MERGE == true:
-// the half-cleaners j < tile of level `kl` (> tile).
-template <class T, int TILE_LOG, bool MERGE>
-__global__ __launch_bounds__(kLocalThreads) void k_sort_local(T* data, uint64_t n,
-                                                              const uint64_t* seg_off,
-                                                              const uint32_t* blk_seg,
-                                                              const uint32_t* blk_first,
-                                                              uint32_t kl) {
+// ---------------------------------------------------------------- tile kernels
+// A tile of 2^tl elements lives in LDS; every stage below the tile size runs there.
+// Stages are applied in register groups: a thread loads the 32 elements that five
+// consecutive stages exchange among themselves (bits b .. b-4 of the index, the top one
+// replaced by the flip mask on a level's first group), applies up to five stages, and
+// writes them back -- one LDS round trip per five stages instead of one per stage.
+// A full tile sort of 2^14 takes 23 round trips (105 stages); a merge takes 3 (14 stages).
+//
+// LDS layout: element i sits at i ^ ((i >> 5) & 63).  For every 32-element group (any b,
+// with or without the flip mask) the 64 lanes of a wave then hit 64 distinct banks with
+// 4-byte elements and 32 distinct 8-byte slots per half wave: no bank conflicts.
+constexpr int kGroupLog = 5;
+constexpr int kGroup = 1 << kGroupLog;
+
+__device__ __forceinline__ uint32_t lds_slot(uint32_t i) { return i ^ ((i >> 5) & 63u); }
+
+// Levels 1..5 of the network on 32 consecutive elements held in registers.
+template <class T>
+__device__ __forceinline__ void sort32(T (&v)[kGroup]) {
+#pragma unroll
+  for (int L = 1; L <= kGroupLog; ++L) {
+    const int hk = 1 << (L - 1);
+#pragma unroll
+    for (int q = 0; q < kGroup; ++q)
+      if (!(q & hk)) cmpx(v[q], v[q ^ ((hk << 1) - 1)]);
+#pragma unroll
+    for (int j = hk >> 1; j >= 1; j >>= 1) {
+#pragma unroll
+      for (int q = 0; q < kGroup; ++q)
+        if (!(q & j)) cmpx(v[q], v[q | j]);
+    }
+  }
+}
+
+// Stages r0..4 of one register group (stage r acts on bit b - r; q's bit 4 - r selects
+// that bit).  FLIP: stage 0 is the level's flip stage, so the elements with q's bit 4 set
+// carry every lower bit inverted and are the upper partner in all later stages.
+template <class T, bool FLIP>
+__device__ __forceinline__ void group_stages(T (&v)[kGroup], int r0) {
+#pragma unroll
+  for (int r = 0; r < kGroupLog; ++r) {
+    if (r < r0) continue;
+#pragma unroll
+    for (int q = 0; q < kGroup; ++q) {
+      if ((q >> (kGroupLog - 1 - r)) & 1) continue;
+      const int q2 = q | (1 << (kGroupLog - 1 - r));
+      const bool q_high = FLIP && r >= 1 && ((q >> (kGroupLog - 1)) & 1);
+      if (!q_high)
+        cmpx(v[q], v[q2]);
+      else
+        cmpx(v[q2], v[q]);
+    }
+  }
+}
+
+// Index of element q of group gid: group bits b-4 .. b, the top one masked by m_top.
+__device__ __forceinline__ uint32_t group_elem(uint32_t irep, int q, uint32_t lowb,
+                                               uint32_t m_top) {
+  return irep ^ (static_cast<uint32_t>(q & (kGroup / 2 - 1)) << lowb) ^
+         ((q & (kGroup / 2)) ? m_top : 0u);
+}
+
+template <class T, bool FLIP>
+__device__ __forceinline__ void lds_group_pass(T* s, uint32_t gid, uint32_t b, uint32_t kl, int r0) {
+  const uint32_t lowb = b - (kGroupLog - 1);
+  const uint32_t irep = ((gid >> lowb) << (b + 1)) | (gid & ((1u << lowb) - 1));
+  const uint32_t m_top = FLIP ? ((1u << kl) - 1) : (1u << b);
+  T v[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) v[q] = s[lds_slot(group_elem(irep, q, lowb, m_top))];
+  group_stages<T, FLIP>(v, r0);
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) s[lds_slot(group_elem(irep, q, lowb, m_top))] = v[q];
+}
+
+// Stages on bits top .. 0 of level kl (the first one the flip stage when flip), in groups
+// of five; the last group is widened down to bit 0 and starts at a later stage.
+template <class T>
+__device__ __forceinline__ void lds_stages(T* s, uint32_t groups, int top, uint32_t kl, bool flip) {
+  int t = top;
+  while (t >= 0) {
+    const int b = t >= kGroupLog - 1 ? t : kGroupLog - 1;
+    const int r0 = b - t;
+    if (threadIdx.x < groups) {
+      if (flip)
+        lds_group_pass<T, true>(s, threadIdx.x, b, kl, r0);
+      else
+        lds_group_pass<T, false>(s, threadIdx.x, b, kl, r0);
+    }
+    __syncthreads();
+    t = b - kGroupLog;
+    flip = false;
+  }
+}
+
+template <class T>
+struct SortCfg {
+  static constexpr int kTileLog = (sizeof(T) == 4) ? 14 : 13;  // 64 KiB of LDS
+  static constexpr int kThreads = 1 << (kTileLog - kGroupLog);  // one group per thread
+  static constexpr int kMaxC = 5;
+};
+
+// MERGE == false: full sort of each tile (levels 1 .. tl).  MERGE == true: the
+// half-cleaners on bits tl-1 .. 0 of level kl (> tile).
+template <class T, bool MERGE>
+__global__ __launch_bounds__(SortCfg<T>::kThreads) void k_sort_local(T* data, uint64_t n,
+                                                                     const uint64_t* seg_off,
+                                                                     const uint32_t* blk_seg,
+                                                                     const uint32_t* blk_first,
+                                                                     uint32_t kl) {
+  constexpr int TL = SortCfg<T>::kTileLog;
+  constexpr uint32_t NT = SortCfg<T>::kThreads;
   extern __shared__ __align__(16) unsigned char s_raw[];
   T* s = reinterpret_cast<T*>(s_raw);
   const SortSeg sg = sort_seg(seg_off, blk_seg, blk_first, n);
-  const uint32_t tl = sg.nplog < TILE_LOG ? sg.nplog : TILE_LOG;
-  const uint32_t te = 1u << tl;
+  const uint32_t tl = sg.nplog < TL ? sg.nplog : TL;
   const uint64_t tbase = static_cast<uint64_t>(sg.lblock) << tl;
   if (tbase >= sg.n || tl == 0) return;
   if (MERGE && sg.nplog < kl) return;  // segment already sorted below this level
+  // tiles shorter than one register group are padded to 32 with +inf in LDS
+  const uint32_t te = tl < kGroupLog ? kGroup : (1u << tl);
   T* d = data + sg.base + tbase;
   const uint64_t lim = sg.n - tbase;  // valid elements of this tile (may exceed te)
+  const uint32_t groups = te >> kGroupLog;
 
-  for (uint32_t i = threadIdx.x; i < te; i += kLocalThreads) s[i] = i < lim ? d[i] : pos_inf<T>();
-  __syncthreads();
-  const uint32_t pairs = te >> 1;
-  if (!MERGE) {
-    for (uint32_t lv = 1; lv <= tl; ++lv) {
-      const uint32_t hk = 1u << (lv - 1);
-      for (uint32_t p = threadIdx.x; p < pairs; p += kLocalThreads) {
-        const uint32_t i = ((p >> (lv - 1)) << lv) | (p & (hk - 1));
-        const uint32_t q = i ^ ((hk << 1) - 1);
-        cmpx(s[i], s[q]);
-      }
-      __syncthreads();
-      for (int jl = static_cast<int>(lv) - 2; jl >= 0; --jl) {
-        const uint32_t j = 1u << jl;
-        for (uint32_t p = threadIdx.x; p < pairs; p += kLocalThreads) {
-          const uint32_t i = ((p >> jl) << (jl + 1)) | (p & (j - 1));
-          cmpx(s[i], s[i + j]);
-        }
-        __syncthreads();
-      }
+  if (te == (1u << TL) && lim >= te) {
+#pragma unroll
+    for (uint32_t k = 0; k < (1u << TL) / NT; ++k) {
+      const uint32_t i = k * NT + threadIdx.x;
+      s[lds_slot(i)] = d[i];
     }
   } else {
-    for (int jl = static_cast<int>(tl) - 1; jl >= 0; --jl) {
-      const uint32_t j = 1u << jl;
-      for (uint32_t p = threadIdx.x; p < pairs; p += kLocalThreads) {
-        const uint32_t i = ((p >> jl) << (jl + 1)) | (p & (j - 1));
-        cmpx(s[i], s[i + j]);
-      }
-      __syncthreads();
-    }
+    for (uint32_t i = threadIdx.x; i < te; i += NT) s[lds_slot(i)] = i < lim ? d[i] : pos_inf<T>();
   }
-  for (uint32_t i = threadIdx.x; i < te; i += kLocalThreads)
-    if (i < lim) d[i] = s[i];
+  __syncthreads();
+  if (!MERGE) {
+    if (threadIdx.x < groups) {
+      T v[kGroup];
+      const uint32_t i0 = threadIdx.x << kGroupLog;
+#pragma unroll
+      for (int q = 0; q < kGroup; ++q) v[q] = s[lds_slot(i0 + q)];
+      sort32<T>(v);
+#pragma unroll
+      for (int q = 0; q < kGroup; ++q) s[lds_slot(i0 + q)] = v[q];
+    }
+    __syncthreads();
+    for (uint32_t L = kGroupLog + 1; L <= tl; ++L) lds_stages<T>(s, groups, static_cast<int>(L) - 1, L, true);
+  } else {
+    lds_stages<T>(s, groups, static_cast<int>(tl) - 1, kl, false);
+  }
+  if (te == (1u << TL) && lim >= te) {
+#pragma unroll
+    for (uint32_t k = 0; k < (1u << TL) / NT; ++k) {
+      const uint32_t i = k * NT + threadIdx.x;
+      d[i] = s[lds_slot(i)];
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < te; i += NT)
+      if (i < lim) d[i] = s[lds_slot(i)];
+  }
 }
 
 // Register pass: C consecutive stages of level kl whose top partner bit is b (the
@@ -194,11 +300,6 @@ __global__ __launch_bounds__(256) void k_sort_global(T* data, uint64_t n, const 
     if (e[q] < lim) d[e[q]] = v[q];
 }
 
-template <class T>
-struct SortCfg {
-  static constexpr int kTileLog = (sizeof(T) == 4) ? 14 : 13;  // 64 KiB of LDS
-  static constexpr int kMaxC = 5;
-};
 
 template <class T>
 hipError_t launch_local(bool merge, uint32_t blocks, T* data, uint64_t n, const uint64_t* seg_off,
@@ -206,10 +307,11 @@ hipError_t launch_local(bool merge, uint32_t blocks, T* data, uint64_t n, const 
                         hipStream_t stream) {
   constexpr int TL = SortCfg<T>::kTileLog;
   const size_t lds = sizeof(T) << TL;
+  constexpr int NT = SortCfg<T>::kThreads;
   if (merge)
-    k_sort_local<T, TL, true><<<blocks, kLocalThreads, lds, stream>>>(data, n, seg_off, blk_seg, blk_first, kl);
+    k_sort_local<T, true><<<blocks, NT, lds, stream>>>(data, n, seg_off, blk_seg, blk_first, kl);
   else
-    k_sort_local<T, TL, false><<<blocks, kLocalThreads, lds, stream>>>(data, n, seg_off, blk_seg, blk_first, kl);
+    k_sort_local<T, false><<<blocks, NT, lds, stream>>>(data, n, seg_off, blk_seg, blk_first, kl);
   return hipGetLastError();
 }
 

@@ -34,7 +34,7 @@ struct SegGeom;
 
 // Launch shapes.  A tile of the tiled kernels is `block` threads x `k` 16-byte vectors
 // per thread.  Defaults can be overridden for sweeps with PARALLAX_TUNE, e.g.
-// "scan_block=512,scan_k=8,scan_bpc=2,red_unroll=8,red_bpc=4,red_nt=1"; only the
+// "scan_block=512,scan_k=8,scan_bpc=2,red_unroll=8,red_bpc=2,red_nt=1"; only the
 // compiled combinations below are accepted (others keep the default).
 struct TileCfg {
   int block;
@@ -45,7 +45,7 @@ struct Tune {
   int scan_bpc = 2;           // resident blocks per CU (0: one tile per block, grid = tiles)
   TileCfg seg_reduce{256, 4};
   int red_unroll = 8;         // 16-byte loads in flight per thread in the plain reduce
-  int red_bpc = 4;            // blocks per CU of the plain reduce
+  int red_bpc = 2;            // blocks per CU of the plain reduce
   int red_nt = 1;             // non-temporal loads in the plain reduce
 };
 Tune& tune();
