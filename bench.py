@@ -132,24 +132,52 @@ def plain_suite(args):
         ("sort:f64", "sort_f64", 2 * n25 * 8),
         ("copy_if:i32", "copy_if_i32", n28 * 4 + kept * 4),
     ]
+    def prof(key):
+        tot = ctypes.c_double()
+        calls = ctypes.c_uint64()
+        if lib.pxr_prof_get(key.encode(), ctypes.byref(tot), ctypes.byref(calls)) != 0:
+            return 0.0, 0
+        return tot.value, calls.value
+
     per = {}
     step_bytes = 0
     for key, name, bytes_ in prims:
-        tot = ctypes.c_double()
-        calls = ctypes.c_uint64()
-        lib.pxr_prof_get(key.encode(), ctypes.byref(tot), ctypes.byref(calls))
-        ms = tot.value / max(calls.value, 1)
+        tot, calls = prof(key)
+        ms = tot / max(calls, 1)
         gbps = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         per[name] = {"ms": round(ms, 4), "alg_bytes": bytes_, "gbps": round(gbps, 1),
                      "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 1)}
         step_bytes += bytes_
+    # Kernels, one launch at a time (live HIP events on the launch stream): the
+    # single-launch primitives, and the three kernels a sort is made of, each of which
+    # reads and writes the whole array once per launch (2*n*s algorithmic bytes).
+    kern = {}
+    for key, name, bytes_ in prims:
+        if not key.startswith("sort"):
+            tot, calls = prof(key)
+            kern[name] = (tot, calls, bytes_)
+    for dt, n, s in (("u32", n26, 4), ("f64", n25, 8)):
+        for k in ("sort_tile", "sort_global", "sort_merge"):
+            tot, calls = prof(f"{k}:{dt}")
+            kern[f"{k}_{dt}"] = (tot, calls, 2 * n * s)
     lib.pxr_prof_enable(0)
+    kernels = {}
+    for name, (tot, calls, bytes_) in kern.items():
+        if calls == 0:
+            continue
+        avg = tot / calls
+        kernels[name] = {"launches_per_step": round(calls / args.steps, 2), "avg_ms": round(avg, 4),
+                         "ms_per_step": round(tot / args.steps, 4), "alg_bytes_per_launch": bytes_,
+                         "gbps": round(bytes_ / (avg * 1e-3) / 1e9, 1)}
     ms_step = wall * 1e3 / args.steps
     value = step_bytes / (ms_step * 1e-3) / 1e9
-    dom = max(per, key=lambda k: per[k]["ms"])
-    roof = {"bound": "hbm", "kernel": dom, "achieved": per[dom]["gbps"], "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s", "frac": round(per[dom]["gbps"] / HBM_PEAK_GBPS, 4), "traffic": None,
-            "units_per_launch": "one call; algorithmic bytes as in per_primitive.alg_bytes"}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    ach = kernels[dom]["gbps"]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+            "traffic": measured_traffic(dom),
+            "units_per_launch": f"{kernels[dom]['alg_bytes_per_launch']} algorithmic bytes per launch",
+            "avg_launch_ms": kernels[dom]["avg_ms"]}
     return {
         "value": round(value, 1), "unit": "GB/s", "ms_per_step": round(ms_step, 3),
         "dtype": "mixed(f32,i64,u32,f64,i32)",
@@ -157,8 +185,25 @@ def plain_suite(args):
                                "reduce 2^28 f32 + 2^28 i64, inclusive + exclusive scan 2^28 u32, "
                                "sort 2^26 u32 + 2^25 f64, copy_if 2^28 i32 (x%2==0), seed 1",
                    "scale": scale},
-        "roofline": roof, "per_primitive": per, "parity": parity,
+        "roofline": roof, "per_primitive": per, "kernels": kernels, "parity": parity,
     }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic.txt: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, scaled
+    by the same counters on a copy of known size), or None when not measured."""
+    f = ROOT / "profiles" / "traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 4 and not line.startswith("#") and parts[0] == kernel:
+            try:
+                return int(float(parts[1]) + float(parts[2]))
+            except ValueError:
+                return None
+    return None
 
 
 def verify(X, D, su, sf, kept):
@@ -301,10 +346,10 @@ def served(args):
     latency under Poisson arrivals at half that rate, and the same requests issued as
     plain per-call synchronous calls (device-resident operands) for the speed-up."""
     import threading
-    from parallax_amd import lib, check, make_desc, consts as C, datagen
+    from parallax_amd import lib, check, make_desc, descriptor_words, consts as C, datagen
     from parallax_amd._lib import PxsConfig, PxsRequest, PxsStats
     from parallax_amd.device import Context
-    from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_U64
+    from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_I64, PX_U64
 
     nreq = max(8, int(400 * args.scale))
     rng_u = datagen.uniform01_f64(4, 4 * nreq, stream=1)
@@ -393,17 +438,78 @@ def served(args):
     lib.pxd_free(out)
     lib.pxd_free(cnt)
     lib.pxs_destroy(server)
+
+    # the reference's own per-call path: the C ABI of parallax/runtime.h with host
+    # operands (each call stages its operand to the device and its result back)
+    def load(d):
+        w, nw = descriptor_words(d)
+        k = lib.parallax_kernel_load(w, nw)
+        assert k, "kernel load"
+        return k
+    K = {}
+    for dt in set(dt_of):
+        K[dt] = {"red": load(make_desc(C.OP_REDUCE, dt)), "scan": load(make_desc(C.OP_SCAN, dt)),
+                 "add": load(make_desc(C.OP_SCAN_ADD, dt)), "sort": load(make_desc(C.OP_SORT, dt)),
+                 "iscan": load(make_desc(C.OP_SCAN, PX_I64)), "iadd": load(make_desc(C.OP_SCAN_ADD, PX_I64)),
+                 "scatter": load(make_desc(C.OP_SCATTER, dt, aux=C.SCATTER_COMPACT))}
+    res = np.zeros(16, dtype=np.uint8)
+
+    def abi_all(work):
+        for (op, dt, a, p, fl), h in zip(reqs, work):
+            k, es = K[dt], a.itemsize
+            if op == C.OP_REDUCE:
+                lib.parallax_reduce(k["red"], h.ctypes.data, h.size, es, res.ctypes.data)
+            elif op == C.OP_SCAN:
+                lib.parallax_scan(k["scan"], k["add"], h.ctypes.data, h.size, es)
+            elif op == C.OP_SORT:
+                lib.parallax_sort(k["sort"], h.ctypes.data, h.size, es)
+            else:
+                o = np.empty_like(h)
+                lib.parallax_copy_if(load_cached(fl), k["iscan"], k["iadd"], k["scatter"],
+                                     h.ctypes.data, o.ctypes.data, h.size, es, 0)
+    flk = {}
+
+    def load_cached(fl):
+        key = id(fl)
+        if key not in flk:
+            flk[key] = load(fl)
+        return flk[key]
+    abi_all([a.copy() for _, _, a, _, _ in reqs])
+    work = [a.copy() for _, _, a, _, _ in reqs]
+    t0 = time.perf_counter()
+    abi_all(work)
+    abi_s = time.perf_counter() - t0
+    abi_rps = nreq / abi_s
     return {"value": round(sat.req_per_s, 1), "unit": "req/s", "ms_per_step": round(1e3 / sat.req_per_s, 4),
             "dtype": "mixed(i32,u64,f32,f64)",
             "config": {"workload": f"served mixed stream: {nreq} requests (30% reduce, 30% scan, 25% sort, "
                                    "15% copy_if), sizes log-uniform [2^10,2^24], dtypes {i32,u64,f32,f64}, "
                                    "seed 4, 64 GB pool, 4 slots", "scale": args.scale},
             "req_per_s": {"served_saturated": round(sat.req_per_s, 1), "served_poisson": round(poi.req_per_s, 1),
-                          "plain_per_call": round(plain_rps, 1)},
+                          "plain_per_call_c_abi_host_operands": round(abi_rps, 1),
+                          "plain_per_call_device_resident": round(plain_rps, 1)},
             "latency_ms": {"poisson_rate_req_s": round(lam, 1), "p50": round(poi.p50_ms, 3), "p99": round(poi.p99_ms, 3),
                            "saturated_p50": round(sat.p50_ms, 3), "saturated_p99": round(sat.p99_ms, 3)},
-            "speedup_vs_plain": round(sat.req_per_s / plain_rps, 2),
+            "speedup_vs_plain": {"c_abi_host_operands": round(sat.req_per_s / abi_rps, 2),
+                                 "device_resident": round(sat.req_per_s / plain_rps, 2)},
             "roofline": None}
+
+
+# ------------------------------------------------------------------ counter reference
+def copy_probe():
+    """Three device-to-device copies of exactly 1 GiB (1 GiB read + 1 GiB written per
+    dispatch): the reference that converts FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    from parallax_amd import lib, check
+    nb = 1 << 30
+    a, b = lib.pxd_malloc(nb), lib.pxd_malloc(nb)
+    assert a and b, "pxd_malloc"
+    check(lib.pxd_memset(a, 1, nb), "memset")
+    for _ in range(3):
+        check(lib.pxd_memcpy_d2d(b, a, nb), "d2d")
+    lib.pxd_device_sync()
+    lib.pxd_free(a)
+    lib.pxd_free(b)
+    log("[bench] copy probe: 3 x 1 GiB device-to-device copies")
 
 
 # ------------------------------------------------------------------ CPU baseline
@@ -509,9 +615,15 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="size multiplier (1.0 = the configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 1 GiB device to device 3 times (the known-size reference that scales "
+                         "FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
     if args.cpu_baseline_child:
         cpu_baseline_child()
+        return
+    if args.copy_probe:
+        copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the primitives run on one device); "
@@ -529,7 +641,7 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": r["dtype"], "data": "synthetic", "config": r["config"],
             "roofline": r.get("roofline"), "cpu_baseline": cpu}
-    for k in ("per_primitive", "parity", "latency_ms", "req_per_s", "speedup_vs_plain"):
+    for k in ("per_primitive", "kernels", "parity", "latency_ms", "req_per_s", "speedup_vs_plain"):
         if k in r:
             line[k] = r[k]
     print(json.dumps(line), flush=True)

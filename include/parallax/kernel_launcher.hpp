@@ -18,6 +18,7 @@
 #include <memory>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "parallax/arena.hpp"
 #include "parallax/descriptor.h"
@@ -74,7 +75,8 @@ class KernelLauncher {
   StreamContext* context() const { return ctx_.get(); }
 
   // Event timing of the device work of each call, on the launcher's stream, keyed
-  // "<op>:<dtype>" (e.g. "scan:u32").  Off by default.
+  // "<op>:<dtype>" (e.g. "scan:u32"); the kernels of a sort are also timed one launch
+  // at a time ("sort_tile:u32", "sort_global:u32", "sort_merge:u32").  Off by default.
   void set_profiling(bool on);
   bool profile(const std::string& key, double* total_ms, uint64_t* calls) const;
 
@@ -86,6 +88,7 @@ class KernelLauncher {
   const px_desc* find(const std::string& name, const char* who) const;
   void prof_begin();
   void prof_end(const char* op, uint32_t dtype);
+  static void prof_hook(void* self, const char* kernel, int phase);
 
   HipBackend* backend_;
   UnifiedArena* arena_;
@@ -96,6 +99,13 @@ class KernelLauncher {
   bool prof_on_ = false;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   std::unordered_map<std::string, std::pair<double, uint64_t>> prof_;
+  struct KernelSpan {
+    const char* kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> ev_pool_;   // per-launch events, reused every call
+  size_t ev_used_ = 0;
+  std::vector<KernelSpan> spans_;     // launches of the current call, resolved at prof_end
 };
 
 }  // namespace parallax

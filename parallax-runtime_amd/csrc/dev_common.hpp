@@ -305,39 +305,70 @@ __device__ __forceinline__ uint32_t peek(const uint64_t* status, uint64_t tile, 
 }
 
 // Decoupled look-back (Merrill & Garland 2016), run by one whole wave for tile `tile`
-// (> first).  Lane L inspects tile (pred - L); tiles before `first` (the first tile of
-// the segment) count as an inclusive prefix of 0.  The wave waits until every lane's
-// tile has published, then sums the aggregates up to and including the closest
-// inclusive prefix.  Returns the exclusive prefix of `tile` in every lane.  Every
-// predecessor has already taken its ticket (tickets are handed out in order), so every
-// wait ends.
+// (> first).  Each poll covers kLookbackR * 64 predecessors: lane L inspects tiles
+// pred - (L*R + r), r = 0..R-1, nearest first, with independent loads, so the inclusive
+// prefix frontier advances 64*R tiles per memory round trip.  Tiles before `first` (the
+// first tile of the segment) count as an inclusive prefix of 0.  The wave stops at the
+// closest inclusive prefix once every tile between it and `tile` has published; it
+// waits on nothing beyond that.  Returns the exclusive prefix of `tile` in every lane.
+// Every predecessor has already taken its ticket (tickets are handed out in order), so
+// every wait ends.
+constexpr int kLookbackR = 4;
+
 template <class A>
 __device__ __forceinline__ A lookback(const uint64_t* status, uint64_t tile, int64_t first,
                                       uint32_t epoch) {
+  constexpr int R = kLookbackR;
   const int lane = lane_id();
   A run = acc_zero<A>();
   int64_t pred = static_cast<int64_t>(tile) - 1;
   while (true) {
-    const int64_t t = pred - lane;
-    A v = acc_zero<A>();
-    uint32_t st = (t >= first) ? peek<A>(status, static_cast<uint64_t>(t), epoch, v) : kStIncl;
-    while (__ballot(st == 0u) != 0ull) {
-      if (st == 0u) {
-        __builtin_amdgcn_s_sleep(1);
-        st = peek<A>(status, static_cast<uint64_t>(t), epoch, v);
+    A v[R];
+    uint32_t st[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t t = pred - (lane * R + r);
+      v[r] = acc_zero<A>();
+      st[r] = (t >= first) ? peek<A>(status, static_cast<uint64_t>(t), epoch, v[r]) : kStIncl;
+    }
+    while (true) {
+      // this lane: first inclusive entry ri (R: none) unless an earlier entry is unpublished
+      int ri = R;
+      bool blocked = false;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (ri == R && !blocked) {
+          if (st[r] == 0u) blocked = true;
+          else if (st[r] == kStIncl) ri = r;
+        }
       }
+      const uint64_t incl = __ballot(ri < R);
+      const uint64_t blk = __ballot(blocked);
+      if (incl != 0ull) {
+        const int p = __ffsll(static_cast<unsigned long long>(incl)) - 1;
+        const uint64_t below = (p == 0) ? 0ull : (~0ull >> (64 - p));
+        if ((blk & below) == 0ull) {
+          A c = acc_zero<A>();
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (lane < p || (lane == p && r <= ri)) c = acc_add(c, v[r]);
+          return acc_add(run, wave_sum(c));
+        }
+      } else if (blk == 0ull) {
+        A c = acc_zero<A>();
+#pragma unroll
+        for (int r = 0; r < R; ++r) c = acc_add(c, v[r]);
+        run = acc_add(run, wave_sum(c));
+        break;  // every entry is an aggregate: next window
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (st[r] == 0u)
+          st[r] = peek<A>(status, static_cast<uint64_t>(pred - (lane * R + r)), epoch, v[r]);
     }
-    const uint64_t incl = __ballot(st == kStIncl);
-    if (incl != 0ull) {
-      const int first = __ffsll(static_cast<unsigned long long>(incl)) - 1;
-      A c = (lane <= first) ? v : acc_zero<A>();
-      run = acc_add(run, wave_sum(c));
-      break;
-    }
-    run = acc_add(run, wave_sum(v));
-    pred -= kWave;
+    pred -= static_cast<int64_t>(kWave) * R;
   }
-  return run;
 }
 
 // ---------------------------------------------------------------- vector IO
@@ -345,5 +376,19 @@ template <class T, int N>
 struct alignas(sizeof(T) * N) Vec {
   T v[N];
 };
+
+using u32x4 = __attribute__((ext_vector_type(4))) unsigned int;
+
+// 16-byte non-temporal load / store of a Vec (streamed data: read or written once).
+template <class V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+  static_assert(sizeof(V) == 16, "16-byte vectors only");
+  return __builtin_bit_cast(V, __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p)));
+}
+template <class V>
+__device__ __forceinline__ void st_stream(V* p, const V& v) {
+  static_assert(sizeof(V) == 16, "16-byte vectors only");
+  __builtin_nontemporal_store(__builtin_bit_cast(u32x4, v), reinterpret_cast<u32x4*>(p));
+}
 
 }  // namespace pxk

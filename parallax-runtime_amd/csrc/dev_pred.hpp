@@ -56,14 +56,67 @@ __device__ __forceinline__ bool pred_eval(uint32_t fn, uint32_t flags, T c0, T c
   return (flags & PX_FLAG_NEGATE) ? !r : r;
 }
 
-// Predicate with its constants unpacked once per kernel.
+// Predicate with its constants unpacked once per kernel.  MOD_EQ by a positive power of
+// two uses a mask: for x < 0 the truncated remainder is (x & (m-1)) - m when nonzero.
 template <class T>
 struct Pred {
   uint32_t fn, flags;
   T c0, c1;
+  T mod_mask;  // m - 1 when MOD_EQ's divisor m is a positive power of two, else 0
   __device__ __forceinline__ Pred(const px_desc& d)
-      : fn(d.fn), flags(d.flags), c0(bits_as<T>(d.c0)), c1(bits_as<T>(d.c1)) {}
+      : fn(d.fn), flags(d.flags), c0(bits_as<T>(d.c0)), c1(bits_as<T>(d.c1)), mod_mask(T(0)) {
+    if constexpr (std::is_integral_v<T>) {
+      if (fn == PX_PRED_MOD_EQ && c0 > T(0) && (c0 & (c0 - T(1))) == T(0)) mod_mask = c0 - T(1);
+    }
+  }
   __device__ __forceinline__ bool operator()(T x) const { return pred_eval<T>(fn, flags, c0, c1, x); }
+
+  // Bit i of the result: predicate of x[i], for i < N (N <= 32), with one branch on the
+  // predicate kind for the whole array.
+  template <int N>
+  __device__ __forceinline__ uint32_t mask(const T (&x)[N]) const {
+    static_assert(N <= 32, "at most 32 elements");
+    uint32_t m = 0;
+#define PXK_PMASK(EXPR)                                     \
+  _Pragma("unroll") for (int i = 0; i < N; ++i) {           \
+    const T v = x[i];                                       \
+    if (EXPR) m |= 1u << i;                                 \
+  }
+    switch (fn) {
+      case PX_PRED_TRUE: m = (N == 32) ? 0xffffffffu : ((1u << N) - 1u); break;
+      case PX_PRED_EQ: PXK_PMASK(v == c0) break;
+      case PX_PRED_NE: PXK_PMASK(v != c0) break;
+      case PX_PRED_LT: PXK_PMASK(v < c0) break;
+      case PX_PRED_LE: PXK_PMASK(v <= c0) break;
+      case PX_PRED_GT: PXK_PMASK(v > c0) break;
+      case PX_PRED_GE: PXK_PMASK(v >= c0) break;
+      case PX_PRED_IN_RANGE: PXK_PMASK((c0 <= v) && (v < c1)) break;
+      case PX_PRED_NONZERO: PXK_PMASK(v != T(0)) break;
+      case PX_PRED_MOD_EQ:
+        if constexpr (std::is_integral_v<T>) {
+          if (mod_mask != T(0)) {
+            const T mm = mod_mask;
+            if constexpr (std::is_signed_v<T>) {
+              PXK_PMASK(((v & mm) != T(0) && v < T(0) ? T((v & mm) - mm - T(1)) : T(v & mm)) == c1)
+            } else {
+              PXK_PMASK(T(v & mm) == c1)
+            }
+            break;
+          }
+        }
+        PXK_PMASK(pred_eval<T>(PX_PRED_MOD_EQ, 0u, c0, c1, v))
+        break;
+      case PX_PRED_BITS_ANY:
+        if constexpr (std::is_integral_v<T>) {
+          PXK_PMASK((v & c0) != T(0))
+        }
+        break;
+      default: break;
+    }
+#undef PXK_PMASK
+    if (flags & PX_FLAG_NEGATE) m = ~m & ((N == 32) ? 0xffffffffu : ((1u << N) - 1u));
+    return m;
+  }
 };
 
 }  // namespace pxk

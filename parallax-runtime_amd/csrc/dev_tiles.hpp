@@ -28,6 +28,8 @@ struct SegGeom {
   uint64_t n;                // element count when seg_lo is null
   uint32_t ntiles;           // total tiles of the launch
   uint32_t nseg;             // 1 when seg_lo is null
+  int tile_block;            // tile shape: threads x 16-byte vectors per thread
+  int tile_k;
 };
 
 struct TileInfo {

@@ -56,7 +56,7 @@ StreamContext::StreamContext(HipBackend* be, hipStream_t s) : be_(be) {
   if (hipMemcpy(ctl_, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return;
   if (hipEventCreateWithFlags(&pin_ev_, hipEventDisableTiming) != hipSuccess) return;
   ok_ = true;
-  if (!reserve(1u << 16, 64u << 10, 1024)) ok_ = false;
+  if (!reserve(1u << 16, 64u << 10)) ok_ = false;
 }
 
 StreamContext::~StreamContext() {
@@ -64,7 +64,6 @@ StreamContext::~StreamContext() {
   if (ctl_) (void)hipFree(ctl_);
   if (status_) (void)hipFree(status_);
   if (partials_) (void)hipFree(partials_);
-  if (segcnt_) (void)hipFree(segcnt_);
   if (work_) (void)hipFree(work_);
   if (pin_) (void)hipHostFree(pin_);
   if (pin_ev_) (void)hipEventDestroy(pin_ev_);
@@ -90,10 +89,9 @@ bool StreamContext::grow(void** p, size_t* cap, size_t need, bool zero) {
   return true;
 }
 
-bool StreamContext::reserve(size_t status_words, size_t partial_bytes, size_t seg_counters) {
+bool StreamContext::reserve(size_t status_words, size_t partial_bytes) {
   if (!grow(reinterpret_cast<void**>(&status_), &status_cap_, status_words * 8, true)) return false;
   if (!grow(&partials_, &partial_cap_, partial_bytes, false)) return false;
-  if (!grow(reinterpret_cast<void**>(&segcnt_), &segcnt_cap_, seg_counters * 4, true)) return false;
   return true;
 }
 
@@ -104,8 +102,6 @@ pxk::Scratch StreamContext::scratch() const {
   s.status_words = status_cap_ / 8;
   s.partials = partials_;
   s.partial_bytes = partial_cap_;
-  s.seg_cnt = segcnt_;
-  s.seg_counters = segcnt_cap_ / 4;
   s.max_blocks = static_cast<uint32_t>(cus_) * 8u;
   return s;
 }
@@ -200,6 +196,8 @@ pxk::SegGeom plain_geom(uint64_t n, int dtype, pxk::TileCfg k, const void* in) {
   g.n = n;
   g.ntiles = static_cast<uint32_t>(nt);
   g.nseg = 1;
+  g.tile_block = k.block;
+  g.tile_k = k.k;
   return g;
 }
 
@@ -244,6 +242,8 @@ int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, pxk::TileCfg k, con
   g.n = 0;
   g.ntiles = tt.ntiles;
   g.nseg = nseg;
+  g.tile_block = k.block;
+  g.tile_k = k.k;
   return PX_OK;
 }
 
@@ -255,7 +255,7 @@ int run_reduce(StreamContext& c, int dtype, const void* in, uint64_t n, void* ou
   ++g_stats.reduce;
   ++g_stats.launches;
   if (n == 0) return zero_dev(c, out_dev, pxk::dtype_size(dtype));
-  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16)) return PX_ERR_NOMEM;
   return dev_err(pxk::reduce(dtype, in, n, out_dev, c.scratch(), c.stream()), "reduce");
 }
 
@@ -265,19 +265,19 @@ int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n,
   if (n == 0) return PX_OK;
   ++g_stats.scan;
   ++g_stats.launches;
-  const pxk::SegGeom g = plain_geom(n, dtype, pxk::tune().scan, in);
-  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0, 0))
+  const pxk::SegGeom g = plain_geom(n, dtype, pxk::tune().scan[dtype], in);
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0))
     return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
   return dev_err(pxk::scan(dtype, in, out, g, exclusive, init_host, c.scratch(), c.stream()), "scan");
 }
 
-int run_sort(StreamContext& c, int dtype, void* data, uint64_t n) {
+int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::LaunchHook& hook) {
   if (!valid_dtype(dtype) || (n && !data)) return PX_ERR_ARG;
   if (n <= 1) return PX_OK;
   ++g_stats.sort;
   ++g_stats.launches;
-  return dev_err(pxk::sort(dtype, data, n, c.stream()), "sort");
+  return dev_err(pxk::sort(dtype, data, n, c.stream(), hook), "sort");
 }
 
 int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
@@ -286,9 +286,9 @@ int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* ou
   ++g_stats.compact;
   ++g_stats.launches;
   if (n == 0) return zero_dev(c, count_dev, sizeof(uint64_t));
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().scan, in);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().compact[flags.dtype], in);
   const size_t words = n >= (1ull << 32) ? 2 : 1;
-  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0, 0)) return PX_ERR_NOMEM;
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
   return dev_err(pxk::compact(flags, PX_SCATTER_COMPACT, in, out, g, count_dev, nullptr,
                               c.scratch(), c.stream()),
@@ -301,12 +301,12 @@ int run_partition(StreamContext& c, const px_desc& flags, const void* in, void* 
   ++g_stats.compact;
   ++g_stats.launches;
   if (n == 0) return zero_dev(c, num_true_dev, sizeof(uint64_t));
-  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16)) return PX_ERR_NOMEM;
   int rc = dev_err(pxk::count_if(flags, in, n, num_true_dev, c.scratch(), c.stream()), "count");
   if (rc) return rc;
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().scan, in);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().compact[flags.dtype], in);
   const size_t words = n >= (1ull << 32) ? 2 : 1;
-  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0, 0)) return PX_ERR_NOMEM;
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
   return dev_err(pxk::compact(flags, PX_SCATTER_PARTITION, in, out, g, nullptr, num_true_dev,
                               c.scratch(), c.stream()),
@@ -342,7 +342,7 @@ int run_argminmax(StreamContext& c, int dtype, const void* in, uint64_t n, bool 
                   bool want_last, uint64_t* out_dev) {
   if (!valid_dtype(dtype) || !out_dev) return PX_ERR_ARG;
   ++g_stats.launches;
-  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16, 0)) return PX_ERR_NOMEM;
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16)) return PX_ERR_NOMEM;
   return dev_err(pxk::argminmax(dtype, in, n, want_max, want_last, out_dev, c.scratch(), c.stream()),
                  "argminmax");
 }
@@ -363,7 +363,7 @@ int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& s
   TileTables tt;
   int rc = seg_geom(c, sp, dtype, pxk::tune().seg_reduce, in, g, tt);
   if (rc) return rc;
-  if (!c.reserve(0, static_cast<size_t>(g.ntiles) * 16, sp.nseg)) return PX_ERR_NOMEM;
+  if (!c.reserve(0, static_cast<size_t>(g.ntiles) * 16)) return PX_ERR_NOMEM;
   return dev_err(pxk::seg_reduce(dtype, in, g, out_dev, c.scratch(), c.stream()), "seg_reduce");
 }
 
@@ -374,9 +374,9 @@ int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const S
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, dtype, pxk::tune().scan, in, g, tt);
+  int rc = seg_geom(c, sp, dtype, pxk::tune().scan[dtype], in, g, tt);
   if (rc) return rc;
-  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0, 0))
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0))
     return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
   return dev_err(pxk::scan(dtype, in, out, g, false, nullptr, c.scratch(), c.stream()), "seg_scan");
@@ -390,9 +390,9 @@ int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, flags.dtype, pxk::tune().scan, in, g, tt);
+  int rc = seg_geom(c, sp, flags.dtype, pxk::tune().compact[flags.dtype], in, g, tt);
   if (rc) return rc;
-  if (!c.reserve(static_cast<size_t>(g.ntiles), 0, 0)) return PX_ERR_NOMEM;
+  if (!c.reserve(static_cast<size_t>(g.ntiles), 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
   return dev_err(pxk::compact(flags, PX_SCATTER_COMPACT, in, out, g, counts_dev, nullptr,
                               c.scratch(), c.stream()),

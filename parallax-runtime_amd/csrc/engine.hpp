@@ -46,7 +46,7 @@ class StreamContext {
   int cu_count() const { return cus_; }
 
   // Grow scratch (synchronises the stream before freeing anything in use).
-  bool reserve(size_t status_words, size_t partial_bytes, size_t seg_counters);
+  bool reserve(size_t status_words, size_t partial_bytes);
   pxk::Scratch scratch() const;
   // Device temp area for per-call tables; contents are valid in stream order until
   // the next call that uses it.  Uploads go through a pinned staging buffer.
@@ -73,8 +73,6 @@ class StreamContext {
   size_t status_cap_ = 0;  // bytes
   void* partials_ = nullptr;
   size_t partial_cap_ = 0;
-  uint32_t* segcnt_ = nullptr;
-  size_t segcnt_cap_ = 0;  // bytes
   void* work_ = nullptr;
   size_t work_cap_ = 0;
   void* pin_ = nullptr;
@@ -109,7 +107,7 @@ void build_tile_tables(const SegSpec& sp, uint32_t tile, uint32_t vec, int64_t s
 int run_reduce(StreamContext& c, int dtype, const void* in, uint64_t n, void* out_dev);
 int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n, bool exclusive,
              const void* init_host);
-int run_sort(StreamContext& c, int dtype, void* data, uint64_t n);
+int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::LaunchHook& hook = {});
 // copy_if / remove_if / unique: count to *count_dev (device u64)
 int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
                 uint64_t* count_dev);

@@ -30,6 +30,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
   constexpr int VEC = 16 / sizeof(T);
   constexpr int TILE = BLOCK * K * VEC;
   constexpr int NW = BLOCK / kWave;
+  constexpr int E = K * VEC;  // elements per thread (<= 32: one flag bit each)
+  static_assert(E <= 32, "flag mask holds 32 elements");
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
 
@@ -42,8 +44,16 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
   const Pred<T> pred(pd);
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
+
+  // Positions inside the tile window are handled as 32-bit offsets rel in [0, TILE):
+  // thread element e = k*VEC + j sits at rel = base_rel(k) + j; the segment covers
+  // [lo_r, hi_r) of the window (lo_r = -1 when the segment starts before it).
+  auto base_rel = [&](int k) { return ((wid * K + k) * kWave + lane) * VEC; };
+
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
 
+  // Persistent loop over tickets; a tile waits only on lower tickets, all of which are
+  // held by running blocks, so every wait ends.
   while (true) {
     if (threadIdx.x == 0)
       s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -52,41 +62,59 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     const uint32_t epoch = s_epoch;
     if (tile >= g.ntiles) break;
     const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
+    const int lo_r = ti.lo - ti.w0 < 0 ? -1 : static_cast<int>(ti.lo - ti.w0);
+    const int hi_r = ti.hi - ti.w0 > TILE ? TILE : static_cast<int>(ti.hi - ti.w0);
+    const bool full = lo_r <= 0 && hi_r == TILE;
+    const V* tp = vin + ti.w0 / VEC;
 
-    T x[K][VEC];
-    uint32_t fl[K];  // bit j: element j of chunk k is kept
+    // ---- raw loads (chunks wholly outside the segment are not read) and valid bits
+    T x[E];
+    uint32_t valid = 0;
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
-      uint32_t valid = 0;
-      if (p0 < ti.hi && p0 + VEC > ti.lo) {
-        const V v = vin[p0 / VEC];
+      for (int k = 0; k < K; ++k) {
+        const V v = ld_stream(tp + base_rel(k) / VEC);
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-          x[k][j] = v.v[j];
-          if (p0 + j >= ti.lo && p0 + j < ti.hi) valid |= 1u << j;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) x[k][j] = T(0);
+        for (int j = 0; j < VEC; ++j) x[k * VEC + j] = v.v[j];
       }
-      uint32_t f = 0;
-      if constexpr (UNIQUE) {
-        // previous element of lane's first item: lane-1's last item, or memory for lane 0
-        T prev = dpp_mov<kWaveShr1, 0xf, 0xf>(x[k][VEC - 1]);
-        if (lane == 0 && p0 - 1 >= ti.lo && p0 - 1 < ti.hi) prev = in_al[p0 - 1];
+      valid = (E == 32) ? 0xffffffffu : ((1u << E) - 1u);
+    } else {
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-          const T pv = (j == 0) ? prev : x[k][j - 1];
-          const bool keep = (p0 + j == ti.lo) || !(x[k][j] == pv);
-          if (((valid >> j) & 1u) && keep) f |= 1u << j;
+      for (int k = 0; k < K; ++k) {
+        const int r0 = base_rel(k);
+        if (r0 < hi_r && r0 + VEC > lo_r) {
+          const V v = tp[r0 / VEC];
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) x[k * VEC + j] = v.v[j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) x[k * VEC + j] = T(0);
         }
-      } else {
 #pragma unroll
         for (int j = 0; j < VEC; ++j)
-          if (((valid >> j) & 1u) && pred(x[k][j])) f |= 1u << j;
+          if (r0 + j >= lo_r && r0 + j < hi_r) valid |= 1u << (k * VEC + j);
       }
-      fl[k] = f;
+    }
+
+    // ---- flags: the predicate (or adjacent difference) on valid positions
+    uint32_t flags;
+    if constexpr (UNIQUE) {
+      flags = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r0 = base_rel(k);
+        // previous element of lane's first item: lane-1's last item, or memory for lane 0
+        T prev = dpp_mov<kWaveShr1, 0xf, 0xf>(x[k * VEC + VEC - 1]);
+        if (lane == 0 && r0 - 1 >= lo_r && r0 - 1 < hi_r) prev = in_al[ti.w0 + r0 - 1];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const T pv = (j == 0) ? prev : x[k * VEC + j - 1];
+          if ((r0 + j == lo_r) || !(x[k * VEC + j] == pv)) flags |= 1u << (k * VEC + j);
+        }
+      }
+      flags &= valid;
+    } else {
+      flags = pred.template mask<E>(x) & valid;
     }
 
     // ---- rank kept elements: per-chunk wave scans of counts
@@ -94,7 +122,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     C tex[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const C c = static_cast<C>(__builtin_popcount(fl[k]));
+      const C c = static_cast<C>(__builtin_popcount((flags >> (k * VEC)) & ((1u << VEC) - 1u)));
       const C incl = wave_incl_scan(c);
       tex[k] = run + wave_shift_right1(incl);
       run += read_lane(incl, kWave - 1);
@@ -106,26 +134,24 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     C tile_kept = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) tile_kept += s_w[j];
-    const int64_t vstart = ti.w0 > ti.lo ? ti.w0 : ti.lo;  // first valid position of the tile
-    const int64_t vend = (ti.w0 + TILE) < ti.hi ? (ti.w0 + TILE) : ti.hi;
-    const C tile_valid = static_cast<C>(vend > vstart ? vend - vstart : 0);
+    const int vstart_r = lo_r > 0 ? lo_r : 0;  // first valid offset of the window
+    const C tile_valid = static_cast<C>(hi_r - vstart_r);
 
     // ---- stage kept (and, for partition, rejected) elements in LDS, in order
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       C r = wave_off + tex[k];
-      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+      const int r0 = base_rel(k);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
-        const bool keep = (fl[k] >> j) & 1u;
-        if (keep) {
-          s_stage[r] = x[k][j];
+        const int e = k * VEC + j;
+        if ((flags >> e) & 1u) {
+          s_stage[r] = x[e];
           ++r;
         } else if (MODE == PX_SCATTER_PARTITION) {
-          const int64_t p = p0 + j;
-          if (p >= ti.lo && p < ti.hi) {
-            const C vi = static_cast<C>(p - vstart);  // valid index inside the tile
-            s_stage[tile_kept + (vi - r)] = x[k][j];
+          if ((valid >> e) & 1u) {
+            const C vi = static_cast<C>(r0 + j - vstart_r);  // valid index inside the tile
+            s_stage[tile_kept + (vi - r)] = x[e];
           }
         }
       }
@@ -160,7 +186,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     for (uint32_t j = threadIdx.x; j < tile_kept; j += BLOCK) o[j] = s_stage[j];
     if (MODE == PX_SCATTER_PARTITION) {
       const uint64_t num_true = *num_true_dev;
-      const uint64_t rej_before = static_cast<uint64_t>(vstart - ti.lo) - P;
+      const uint64_t rej_before = static_cast<uint64_t>(ti.w0 + vstart_r - ti.lo) - P;
       T* orj = out + seg_out + num_true + rej_before;
       for (uint32_t j = tile_kept + threadIdx.x; j < tile_valid; j += BLOCK)
         orj[j - tile_kept] = s_stage[j];
@@ -209,12 +235,8 @@ template <class T, class C, int MODE, bool UNIQUE>
 hipError_t launch_compact_t(hipStream_t stream, const T* in_al, int64_t in_shift, T* out,
                             const SegGeom& g, const px_desc& pd, uint64_t* counts,
                             const uint64_t* num_true_dev, const Scratch& s) {
-  const TileCfg c = tune().scan;
-  uint32_t grid = g.ntiles;
-  if (tune().scan_bpc > 0) {
-    const uint32_t cap = s.max_blocks / 8 * static_cast<uint32_t>(tune().scan_bpc);
-    if (grid > cap) grid = cap;
-  }
+  const TileCfg c{g.tile_block, g.tile_k};
+  const uint32_t grid = lookback_grid(g.ntiles, s.max_blocks, c.block);
   if (grid == 0) return hipSuccess;
 #define PXK_CMP(B, KK)                                                                        \
   if (c.block == B && c.k == KK) {                                                            \
@@ -222,10 +244,10 @@ hipError_t launch_compact_t(hipStream_t stream, const T* in_al, int64_t in_shift
         in_al, in_shift, out, g, pd, counts, num_true_dev, s.status, s.ctl);                  \
     return hipGetLastError();                                                                 \
   }
-  PXK_CMP(256, 4)
   PXK_CMP(256, 8)
-  PXK_CMP(512, 4)
   PXK_CMP(512, 8)
+  PXK_CMP(1024, 4)
+  PXK_CMP(1024, 8)
 #undef PXK_CMP
   return hipErrorInvalidValue;
 }

@@ -21,7 +21,6 @@ namespace pxk {
 
 namespace {
 
-using u32x4 = __attribute__((ext_vector_type(4))) unsigned int;
 
 // 16-byte load, optionally non-temporal (streamed data is read once: keep it out of the
 // caches the look-back and partials use).
@@ -113,18 +112,20 @@ __global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int
   }
 }
 
+// Segmented reduce, pass 1: tile partials.  Tiles are independent (static grid-stride
+// assignment, no atomics); a segment held by one tile writes its result directly, the
+// others leave one partial per tile for pass 2.
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al, int64_t shift,
                                                     SegGeom g,
                                                     typename Traits<T>::A* __restrict__ partials,
-                                                    uint32_t* seg_cnt, T* __restrict__ out) {
+                                                    T* __restrict__ out) {
   using A = typename Traits<T>::A;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int TILE = 256 * K * VEC;
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
   __shared__ A s_w[4];
-  __shared__ int s_last;
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
 
@@ -146,40 +147,37 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
         }
       }
     }
-    A w = wave_sum(acc);
+    const A w = wave_sum(acc);
     if (lane == 0) s_w[wid] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
       const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
-      s_last = 0;
-      if (ti.first_tile == ti.last_tile) {
+      if (ti.first_tile == ti.last_tile)
         out[g.seg_out ? g.seg_out[ti.seg] : ti.seg] = from_acc<T>(tot);
-      } else {
-        st_carry(&partials[tile], tot);
-        const uint32_t d = __hip_atomic_fetch_add(&seg_cnt[ti.seg], 1u, __ATOMIC_ACQ_REL,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        if (d == ti.last_tile - ti.first_tile) {
-          st_agent32(&seg_cnt[ti.seg], 0u);
-          s_last = 1;
-        }
-      }
-    }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      A c = acc_zero<A>();
-      for (uint32_t j = ti.first_tile + threadIdx.x; j <= ti.last_tile; j += 256)
-        c = acc_add(c, ld_carry(&partials[j]));
-      w = wave_sum(c);
-      __syncthreads();
-      if (lane == 0) s_w[wid] = w;
-      __syncthreads();
-      if (threadIdx.x == 0)
-        out[g.seg_out ? g.seg_out[ti.seg] : ti.seg] =
-            from_acc<T>(acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3])));
+      else
+        partials[tile] = tot;
     }
     __syncthreads();
   }
+}
+
+// Segmented reduce, pass 2: one wave per segment of more than one tile sums its tile
+// partials in a fixed order (lane-strided, then the wave tree), so results do not
+// depend on scheduling.
+template <class T>
+__global__ __launch_bounds__(256) void k_seg_combine(SegGeom g,
+                                                     const typename Traits<T>::A* __restrict__ partials,
+                                                     T* __restrict__ out) {
+  using A = typename Traits<T>::A;
+  const uint32_t seg = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (seg >= g.nseg) return;
+  const uint32_t first = g.tile_off[seg];
+  const uint32_t last = g.tile_off[seg + 1] - 1;
+  if (last == first) return;  // written by pass 1
+  A c = acc_zero<A>();
+  for (uint32_t j = first + lane_id(); j <= last; j += kWave) c = acc_add(c, partials[j]);
+  const A w = wave_sum(c);
+  if (lane_id() == 0) out[g.seg_out ? g.seg_out[seg] : seg] = from_acc<T>(w);
 }
 
 template <class T>
@@ -219,11 +217,13 @@ hipError_t seg_reduce_t(const void* in, const SegGeom& g, void* out, const Scrat
   const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
   const int64_t shift = elem_shift(in, sizeof(T));
   if (static_cast<size_t>(g.ntiles) * sizeof(A) > s.partial_bytes) return hipErrorInvalidValue;
-  if (g.nseg > s.seg_counters) return hipErrorInvalidValue;
   uint32_t grid = g.ntiles < s.max_blocks ? g.ntiles : s.max_blocks;
   if (grid == 0) return hipSuccess;
-  k_seg_reduce<T, kSegReduceK><<<grid, 256, 0, stream>>>(
-      in_al, shift, g, reinterpret_cast<A*>(s.partials), s.seg_cnt, reinterpret_cast<T*>(out));
+  A* part = reinterpret_cast<A*>(s.partials);
+  T* o = reinterpret_cast<T*>(out);
+  k_seg_reduce<T, kSegReduceK><<<grid, 256, 0, stream>>>(in_al, shift, g, part, o);
+  if (g.ntiles > g.nseg)  // some segment spans more than one tile
+    k_seg_combine<T><<<(g.nseg + 3) / 4, 256, 0, stream>>>(g, part, o);
   return hipGetLastError();
 }
 

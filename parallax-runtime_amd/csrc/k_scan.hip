@@ -6,10 +6,10 @@
 // synchronised dispatches at 2^28) and the exclusive shift pass (launch_exclusive_scan,
 // :1437-1475) with ONE launch that reads every element once and writes it once (2*n*s).
 //
-// Tile = 256 threads x K chunks x one 16-byte vector per lane.  Wave w owns K
+// Tile = BLOCK threads x K chunks x one 16-byte vector per lane.  Wave w owns K
 // consecutive 64-vector chunks; per chunk each lane sums its vector, the wave scans the
-// 64 sums with DPP and carries the running total to the next chunk; the four wave
-// totals meet in LDS.  Tiles take tickets in order; each publishes its aggregate, wave 0
+// 64 sums with DPP and carries the running total to the next chunk; the wave totals meet
+// in LDS.  Tiles take tickets in order; each publishes its aggregate, wave 0
 // looks back over up to 64 predecessors per step (decoupled look-back), publishes its
 // inclusive prefix, and the tile writes its outputs with 16-byte stores.  Tiles of a
 // batched (segmented) call look back only inside their own segment.
@@ -43,8 +43,41 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
 
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
+
+  // Masked tile load (zeros outside the tile's segment window).
+  auto load = [&](T (&dst)[K][VEC], const TileInfo& t) {
+    if (t.w0 >= t.lo && t.w0 + TILE <= t.hi) {  // whole tile inside: no masks
+      const V* tp = vin + t.w0 / VEC + wid * (K * kWave) + lane;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const V v = ld_stream(tp + k * kWave);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) dst[k][j] = v.v[j];
+      }
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t p0 = t.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+      if (p0 >= t.lo && p0 + VEC <= t.hi) {
+        const V v = vin[p0 / VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) dst[k][j] = v.v[j];
+      } else if (p0 < t.hi && p0 + VEC > t.lo) {
+        const V v = vin[p0 / VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) dst[k][j] = (p0 + j >= t.lo && p0 + j < t.hi) ? v.v[j] : T(0);
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) dst[k][j] = T(0);
+      }
+    }
+  };
+
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
 
+  // Persistent loop over tickets; a tile waits only on lower tickets, all of which are
+  // held by running blocks, so every wait ends.
   while (true) {
     if (threadIdx.x == 0)
       s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -53,25 +86,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     const uint32_t epoch = s_epoch;
     if (tile >= g.ntiles) break;
     const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
-
-    // ---- load (masked outside the segment)
     T x[K][VEC];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
-      if (p0 >= ti.lo && p0 + VEC <= ti.hi) {
-        const V v = vin[p0 / VEC];
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) x[k][j] = v.v[j];
-      } else if (p0 < ti.hi && p0 + VEC > ti.lo) {
-        const V v = vin[p0 / VEC];
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) x[k][j] = (p0 + j >= ti.lo && p0 + j < ti.hi) ? v.v[j] : T(0);
-      } else {
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) x[k][j] = T(0);
-      }
-    }
+    load(x, ti);
 
     // ---- per-chunk wave scans, carried across the wave's chunks
     A run = acc_zero<A>();
@@ -130,7 +146,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
       }
       const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
       if (vec_ok && p0 >= ti.lo && p0 + VEC <= ti.hi) {
-        vout[(p0 + odelta + out_shift) / VEC] = y;
+        st_stream(&vout[(p0 + odelta + out_shift) / VEC], y);
       } else if (p0 < ti.hi && p0 + VEC > ti.lo) {
 #pragma unroll
         for (int j = 0; j < VEC; ++j)
@@ -163,12 +179,8 @@ hipError_t scan_t(const void* in, void* out, const SegGeom& g, bool exclusive, c
       iv = static_cast<A>(t);
     }
   }
-  const TileCfg c = tune().scan;
-  uint32_t grid = g.ntiles;
-  if (tune().scan_bpc > 0) {
-    const uint32_t cap = s.max_blocks / 8 * static_cast<uint32_t>(tune().scan_bpc);
-    if (grid > cap) grid = cap;
-  }
+  const TileCfg c{g.tile_block, g.tile_k};
+  const uint32_t grid = lookback_grid(g.ntiles, s.max_blocks, c.block);
   if (grid == 0) return hipSuccess;
   T* o = reinterpret_cast<T*>(out);
 #define PXK_SCAN(B, KK)                                                                            \
@@ -179,10 +191,10 @@ hipError_t scan_t(const void* in, void* out, const SegGeom& g, bool exclusive, c
       k_scan<T, B, KK, false><<<grid, B, 0, stream>>>(in_al, in_shift, o, out_shift, g, iv, s.status, s.ctl); \
     return hipGetLastError();                                                                      \
   }
-  PXK_SCAN(256, 4)
   PXK_SCAN(256, 8)
-  PXK_SCAN(512, 4)
   PXK_SCAN(512, 8)
+  PXK_SCAN(1024, 4)
+  PXK_SCAN(1024, 8)
 #undef PXK_SCAN
   return hipErrorInvalidValue;
 }

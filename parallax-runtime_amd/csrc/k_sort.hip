@@ -22,6 +22,7 @@
 // different lengths sort in the same few launches (each segment virtually padded to its
 // own power of two).
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "dev_common.hpp"
@@ -46,13 +47,22 @@ __device__ __forceinline__ float pos_inf<float>() { return __builtin_huge_valf()
 template <>
 __device__ __forceinline__ double pos_inf<double>() { return __builtin_huge_val(); }
 
-// min to a, max to b; a swap only when b < a (operator<, as std::sort)
+// min to a, max to b; a swap only when b < a (operator<, as std::sort).  Integers use
+// min/max (2 instructions); floats keep compare + select, so NaN and -0.0 inputs still
+// come out as a permutation of the input bits.
 template <class T>
 __device__ __forceinline__ void cmpx(T& a, T& b) {
-  if (b < a) {
-    const T t = a;
-    a = b;
-    b = t;
+  if constexpr (std::is_integral_v<T>) {
+    const T lo = b < a ? b : a;
+    const T hi = b < a ? a : b;
+    a = lo;
+    b = hi;
+  } else {
+    if (b < a) {
+      const T t = a;
+      a = b;
+      b = t;
+    }
   }
 }
 
@@ -140,39 +150,80 @@ __device__ __forceinline__ void group_stages(T (&v)[kGroup], int r0) {
   }
 }
 
-// Index of element q of group gid: group bits b-4 .. b, the top one masked by m_top.
-__device__ __forceinline__ uint32_t group_elem(uint32_t irep, int q, uint32_t lowb,
-                                               uint32_t m_top) {
-  return irep ^ (static_cast<uint32_t>(q & (kGroup / 2 - 1)) << lowb) ^
-         ((q & (kGroup / 2)) ? m_top : 0u);
+// Group pass with the group's top bit B known at compile time.  The LDS slot map is
+// linear over XOR (slot(a ^ b) = slot(a) ^ slot(b)), and every element of the group is
+// irep ^ c(q) with a constant c(q), so each slot is slot(irep) ^ slot(c(q)): one XOR per
+// element.  FLIP groups are a level's first group (kl = B + 1).
+__host__ __device__ constexpr uint32_t group_const(int q, int B, bool flip) {
+  return (static_cast<uint32_t>(q & (kGroup / 2 - 1)) << (B - (kGroupLog - 1))) ^
+         ((q & (kGroup / 2)) ? (flip ? ((2u << B) - 1u) : (1u << B)) : 0u);
+}
+
+template <class T, int B, bool FLIP>
+__device__ __attribute__((noinline)) void lds_group_pass(T* s, uint32_t gid, int r0) {
+  constexpr uint32_t LOWB = B - (kGroupLog - 1);
+  const uint32_t irep = ((gid >> LOWB) << (B + 1)) | (gid & ((1u << LOWB) - 1u));
+  const uint32_t base = lds_slot(irep);
+  T v[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) v[q] = s[base ^ lds_slot(group_const(q, B, FLIP))];
+  group_stages<T, FLIP>(v, r0);
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) s[base ^ lds_slot(group_const(q, B, FLIP))] = v[q];
+}
+
+// Group pass with the geometry at run time (8-byte elements: the compile-time passes
+// above cost more in call overhead and registers than they save in address math).
+template <class T, bool FLIP>
+__device__ __forceinline__ void lds_group_pass_rt(T* s, uint32_t gid, uint32_t b, int r0) {
+  const uint32_t lowb = b - (kGroupLog - 1);
+  const uint32_t irep = ((gid >> lowb) << (b + 1)) | (gid & ((1u << lowb) - 1u));
+  const uint32_t m_top = FLIP ? ((2u << b) - 1u) : (1u << b);
+  T v[kGroup];
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) {
+    const uint32_t e = irep ^ (static_cast<uint32_t>(q & (kGroup / 2 - 1)) << lowb) ^
+                       ((q & (kGroup / 2)) ? m_top : 0u);
+    v[q] = s[lds_slot(e)];
+  }
+  group_stages<T, FLIP>(v, r0);
+#pragma unroll
+  for (int q = 0; q < kGroup; ++q) {
+    const uint32_t e = irep ^ (static_cast<uint32_t>(q & (kGroup / 2 - 1)) << lowb) ^
+                       ((q & (kGroup / 2)) ? m_top : 0u);
+    s[lds_slot(e)] = v[q];
+  }
 }
 
 template <class T, bool FLIP>
-__device__ __forceinline__ void lds_group_pass(T* s, uint32_t gid, uint32_t b, uint32_t kl, int r0) {
-  const uint32_t lowb = b - (kGroupLog - 1);
-  const uint32_t irep = ((gid >> lowb) << (b + 1)) | (gid & ((1u << lowb) - 1));
-  const uint32_t m_top = FLIP ? ((1u << kl) - 1) : (1u << b);
-  T v[kGroup];
-#pragma unroll
-  for (int q = 0; q < kGroup; ++q) v[q] = s[lds_slot(group_elem(irep, q, lowb, m_top))];
-  group_stages<T, FLIP>(v, r0);
-#pragma unroll
-  for (int q = 0; q < kGroup; ++q) s[lds_slot(group_elem(irep, q, lowb, m_top))] = v[q];
+__device__ __forceinline__ void lds_group_dispatch(T* s, uint32_t gid, int b, int r0) {
+  if constexpr (sizeof(T) == 8) {
+    lds_group_pass_rt<T, FLIP>(s, gid, static_cast<uint32_t>(b), r0);
+    return;
+  }
+  switch (b) {
+#define PXK_GB(BB) \
+  case BB: lds_group_pass<T, BB, FLIP>(s, gid, r0); break;
+    PXK_GB(4) PXK_GB(5) PXK_GB(6) PXK_GB(7) PXK_GB(8)
+    PXK_GB(9) PXK_GB(10) PXK_GB(11) PXK_GB(12) PXK_GB(13)
+#undef PXK_GB
+    default: break;
+  }
 }
 
-// Stages on bits top .. 0 of level kl (the first one the flip stage when flip), in groups
-// of five; the last group is widened down to bit 0 and starts at a later stage.
+// Stages on bits top .. 0 of one level (the first one the flip stage when flip), in
+// groups of five; the last group is widened down to bit 0 and starts at a later stage.
 template <class T>
-__device__ __forceinline__ void lds_stages(T* s, uint32_t groups, int top, uint32_t kl, bool flip) {
+__device__ __forceinline__ void lds_stages(T* s, uint32_t groups, int top, bool flip) {
   int t = top;
   while (t >= 0) {
     const int b = t >= kGroupLog - 1 ? t : kGroupLog - 1;
     const int r0 = b - t;
     if (threadIdx.x < groups) {
       if (flip)
-        lds_group_pass<T, true>(s, threadIdx.x, b, kl, r0);
+        lds_group_dispatch<T, true>(s, threadIdx.x, b, r0);
       else
-        lds_group_pass<T, false>(s, threadIdx.x, b, kl, r0);
+        lds_group_dispatch<T, false>(s, threadIdx.x, b, r0);
     }
     __syncthreads();
     t = b - kGroupLog;
@@ -231,9 +282,9 @@ __global__ __launch_bounds__(SortCfg<T>::kThreads) void k_sort_local(T* data, ui
       for (int q = 0; q < kGroup; ++q) s[lds_slot(i0 + q)] = v[q];
     }
     __syncthreads();
-    for (uint32_t L = kGroupLog + 1; L <= tl; ++L) lds_stages<T>(s, groups, static_cast<int>(L) - 1, L, true);
+    for (uint32_t L = kGroupLog + 1; L <= tl; ++L) lds_stages<T>(s, groups, static_cast<int>(L) - 1, true);
   } else {
-    lds_stages<T>(s, groups, static_cast<int>(tl) - 1, kl, false);
+    lds_stages<T>(s, groups, static_cast<int>(tl) - 1, false);
   }
   if (te == (1u << TL) && lim >= te) {
 #pragma unroll
@@ -345,7 +396,7 @@ inline uint32_t host_ceil_log2(uint64_t n) {
 }
 
 template <class T>
-hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream) {
+hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream, const LaunchHook& hook) {
   T* data = reinterpret_cast<T*>(data_v);
   if (n <= 1) return hipSuccess;
   constexpr int TL = SortCfg<T>::kTileLog;
@@ -353,8 +404,10 @@ hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream) {
   const uint32_t tl = nplog < TL ? nplog : TL;
   // tiles that hold at least one real element
   const uint64_t ntiles = (n + (1ull << tl) - 1) >> tl;
+  hook("sort_tile", 0);
   hipError_t e = launch_local<T>(false, static_cast<uint32_t>(ntiles), data, n, nullptr, nullptr,
                                  nullptr, 0, stream);
+  hook("sort_tile", 1);
   if (e != hipSuccess) return e;
   for (uint32_t kl = tl + 1; kl <= nplog; ++kl) {
     int remaining = static_cast<int>(kl - tl);
@@ -364,15 +417,19 @@ hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream) {
       const int c = std::min(remaining, SortCfg<T>::kMaxC);
       const uint64_t groups = (1ull << nplog) >> c;
       const uint64_t blocks = (groups + 255) / 256;
+      hook("sort_global", 0);
       e = launch_global<T>(c, first, static_cast<uint32_t>(blocks), data, n, nullptr, nullptr,
                            nullptr, b, kl, stream);
+      hook("sort_global", 1);
       if (e != hipSuccess) return e;
       b -= c;
       remaining -= c;
       first = false;
     }
+    hook("sort_merge", 0);
     e = launch_local<T>(true, static_cast<uint32_t>(ntiles), data, n, nullptr, nullptr, nullptr, kl,
                         stream);
+    hook("sort_merge", 1);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -468,8 +525,8 @@ hipError_t seg_sort_run(const SegSortPlan& p, void* data, const uint64_t* seg_of
   return hipErrorInvalidValue;
 }
 
-hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream) {
-  PXK_DISPATCH(dtype, return sort_t<T>(data, n, stream));
+hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream, const LaunchHook& hook) {
+  PXK_DISPATCH(dtype, return sort_t<T>(data, n, stream, hook));
   return hipErrorInvalidValue;
 }
 

@@ -32,24 +32,29 @@ constexpr uint32_t kEpochMax = (1u << 30) - 1;
 
 struct SegGeom;
 
-// Launch shapes.  A tile of the tiled kernels is `block` threads x `k` 16-byte vectors
-// per thread.  Defaults can be overridden for sweeps with PARALLAX_TUNE, e.g.
-// "scan_block=512,scan_k=8,scan_bpc=2,red_unroll=8,red_bpc=2,red_nt=1"; only the
-// compiled combinations below are accepted (others keep the default).
+// Launch shapes.  A tile of the look-back kernels (scan, compaction) is `block` threads x
+// `k` 16-byte vectors per thread; the shape is chosen per element type (registers) and
+// recorded in the geometry, so the tile count computed on the host and the tile size
+// the kernel uses always agree.  Defaults can be overridden for sweeps with
+// PARALLAX_TUNE, e.g. "scan_block=1024,scan_k=8,compact_block=1024,compact_k=8,bpc=1,
+// red_unroll=8,red_bpc=1,red_nt=1"; only the compiled shapes below are accepted.
 struct TileCfg {
   int block;
   int k;
 };
 struct Tune {
-  TileCfg scan{512, 8};       // scan and compaction (look-back kernels)
-  int scan_bpc = 2;           // resident blocks per CU (0: one tile per block, grid = tiles)
+  TileCfg scan[PX_NUM_DTYPES];     // per element type
+  TileCfg compact[PX_NUM_DTYPES];
+  int bpc = 0;                // resident blocks per CU of the look-back kernels (0: 1024/block)
   TileCfg seg_reduce{256, 4};
   int red_unroll = 8;         // 16-byte loads in flight per thread in the plain reduce
-  int red_bpc = 2;            // blocks per CU of the plain reduce
+  int red_bpc = 1;            // blocks per CU of the plain reduce
   int red_nt = 1;             // non-temporal loads in the plain reduce
 };
 Tune& tune();
 bool tile_cfg_supported(TileCfg c);
+// Persistent grid of a look-back launch: every tile, at most the resident blocks.
+uint32_t lookback_grid(uint32_t ntiles, uint32_t max_blocks, int block);
 constexpr int kSegReduceK = 4;
 
 // Scratch of one stream context.  All of it is device memory; ctl is zeroed once with
@@ -60,8 +65,6 @@ struct Scratch {
   size_t status_words = 0;
   void* partials = nullptr;     // per-block / per-tile partial sums
   size_t partial_bytes = 0;
-  uint32_t* seg_cnt = nullptr;  // per-segment counters (self-resetting)
-  size_t seg_counters = 0;
   uint32_t max_blocks = 2048;   // resident-block cap (CUs x 8)
 };
 
@@ -119,8 +122,20 @@ hipError_t compact(const px_desc& pred, int mode, const void* in, void* out, con
 hipError_t count_if(const px_desc& pred, const void* in, uint64_t n, uint64_t* count_dev,
                     const Scratch& s, hipStream_t stream);
 
-// ---- sort (k_sort.hip): ascending; positions past n act as +infinity.
-hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream);
+// Optional per-launch hook of multi-launch primitives: fn(ctx, kernel, 0) right before
+// and fn(ctx, kernel, 1) right after each kernel launch (used for per-kernel timing).
+struct LaunchHook {
+  void (*fn)(void* ctx, const char* kernel, int phase) = nullptr;
+  void* ctx = nullptr;
+  void operator()(const char* kernel, int phase) const {
+    if (fn) fn(ctx, kernel, phase);
+  }
+};
+
+// ---- sort (k_sort.hip): ascending; positions past n act as +infinity.  Kernels:
+// "sort_tile" (LDS sort of each tile), "sort_global" (register pass over HBM) and
+// "sort_merge" (LDS half-cleaners of one level); each reads and writes the array once.
+hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream, const LaunchHook& hook = {});
 // Segmented sort of [seg_off[s], seg_off[s+1]) for every segment: a host plan (block
 // tables for each launch kind) uploaded by the caller, then a fixed launch sequence.
 struct SegSortPlan {

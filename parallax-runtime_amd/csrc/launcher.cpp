@@ -66,6 +66,7 @@ KernelLauncher::~KernelLauncher() {
   if (ctx_) (void)hipStreamSynchronize(ctx_->stream());
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   if (dev_word_) (void)hipFree(dev_word_);
   if (dev_value_) (void)hipFree(dev_value_);
 }
@@ -165,7 +166,31 @@ bool KernelLauncher::profile(const std::string& key, double* total_ms, uint64_t*
 }
 
 void KernelLauncher::prof_begin() {
+  spans_.clear();
+  ev_used_ = 0;
   if (prof_on_ && ev0_) (void)hipEventRecord(ev0_, ctx_->stream());
+}
+
+// Records an event before (phase 0) and after (phase 1) one kernel launch of the call.
+// Back-to-back launches share one event (the end of one is the start of the next), so
+// the stream carries one marker per launch.
+void KernelLauncher::prof_hook(void* self, const char* kernel, int phase) {
+  auto* L = static_cast<KernelLauncher*>(self);
+  if (phase == 0 && !L->spans_.empty() && L->spans_.back().b) {
+    L->spans_.push_back({kernel, L->spans_.back().b, nullptr});
+    return;
+  }
+  if (L->ev_used_ == L->ev_pool_.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    L->ev_pool_.push_back(e);
+  }
+  hipEvent_t e = L->ev_pool_[L->ev_used_++];
+  (void)hipEventRecord(e, L->ctx_->stream());
+  if (phase == 0)
+    L->spans_.push_back({kernel, e, nullptr});
+  else if (!L->spans_.empty())
+    L->spans_.back().b = e;
 }
 
 void KernelLauncher::prof_end(const char* op, uint32_t dtype) {
@@ -175,9 +200,19 @@ void KernelLauncher::prof_end(const char* op, uint32_t dtype) {
   if (hipEventSynchronize(ev1_) != hipSuccess) return;
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, ev0_, ev1_) != hipSuccess) return;
-  auto& e = prof_[std::string(op) + ":" + (dtype < 6 ? names[dtype] : "?")];
+  const std::string dt = dtype < 6 ? names[dtype] : "?";
+  auto& e = prof_[std::string(op) + ":" + dt];
   e.first += ms;
   e.second += 1;
+  for (const KernelSpan& k : spans_) {
+    float kms = 0.f;
+    if (!k.b || hipEventElapsedTime(&kms, k.a, k.b) != hipSuccess) continue;
+    auto& ke = prof_[std::string(k.kernel) + ":" + dt];
+    ke.first += kms;
+    ke.second += 1;
+  }
+  spans_.clear();
+  ev_used_ = 0;
 }
 
 namespace {
@@ -357,7 +392,9 @@ bool KernelLauncher::launch_sort(const std::string& kernel_name, void* data, siz
   bool ok = resolve(data, count * elem_size, true, op);
   if (ok) {
     prof_begin();
-    ok = run_sort(*ctx_, d->dtype, op.dev, count) == PX_OK;
+    pxk::LaunchHook hook;
+    if (prof_on_) hook = pxk::LaunchHook{&KernelLauncher::prof_hook, this};
+    ok = run_sort(*ctx_, d->dtype, op.dev, count, hook) == PX_OK;
     prof_end("sort", d->dtype);
   }
   ok = finish(op, ok) && ok;

@@ -8,16 +8,36 @@
 namespace pxk {
 
 bool tile_cfg_supported(TileCfg c) {
-  return (c.block == 256 || c.block == 512) && (c.k == 4 || c.k == 8);
+  return (c.block == 256 && c.k == 8) || (c.block == 512 && c.k == 8) ||
+         (c.block == 1024 && (c.k == 4 || c.k == 8));
+}
+
+uint32_t lookback_grid(uint32_t ntiles, uint32_t max_blocks, int block) {
+  const uint32_t cus = max_blocks / 8;
+  const int bpc = tune().bpc > 0 ? tune().bpc : (1024 / block > 0 ? 1024 / block : 1);
+  const uint32_t cap = cus * static_cast<uint32_t>(bpc);
+  return ntiles < cap ? ntiles : cap;
 }
 
 namespace {
-Tune load() {
+Tune defaults() {
   Tune t;
+  for (int d = 0; d < PX_NUM_DTYPES; ++d) {
+    // 1024 x 8 keeps integer and f32 scans within 128 VGPRs; the double-double carry of
+    // f64 needs the 1024 x 4 tile.
+    t.scan[d] = (d == PX_F64) ? TileCfg{1024, 4} : TileCfg{1024, 8};
+    t.compact[d] = TileCfg{1024, 8};
+  }
+  return t;
+}
+
+Tune load() {
+  Tune t = defaults();
   const char* env = std::getenv("PARALLAX_TUNE");
   if (!env) return t;
   std::string s(env);
   size_t pos = 0;
+  TileCfg scan{0, 0}, compact{0, 0};
   while (pos < s.size()) {
     size_t comma = s.find(',', pos);
     if (comma == std::string::npos) comma = s.size();
@@ -27,14 +47,20 @@ Tune load() {
     if (eq == std::string::npos) continue;
     const std::string k = kv.substr(0, eq);
     const int v = std::atoi(kv.c_str() + eq + 1);
-    TileCfg scan = t.scan;
     if (k == "scan_block") scan.block = v;
     else if (k == "scan_k") scan.k = v;
-    else if (k == "scan_bpc") t.scan_bpc = v;
+    else if (k == "compact_block") compact.block = v;
+    else if (k == "compact_k") compact.k = v;
+    else if (k == "bpc" || k == "scan_bpc") t.bpc = v;
     else if (k == "red_unroll") t.red_unroll = v;
     else if (k == "red_bpc") t.red_bpc = v;
     else if (k == "red_nt") t.red_nt = v;
-    if (tile_cfg_supported(scan)) t.scan = scan;
+  }
+  for (int d = 0; d < PX_NUM_DTYPES; ++d) {
+    TileCfg a{scan.block ? scan.block : t.scan[d].block, scan.k ? scan.k : t.scan[d].k};
+    if (tile_cfg_supported(a)) t.scan[d] = a;
+    TileCfg b{compact.block ? compact.block : t.compact[d].block, compact.k ? compact.k : t.compact[d].k};
+    if (tile_cfg_supported(b)) t.compact[d] = b;
   }
   return t;
 }
