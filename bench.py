@@ -274,6 +274,12 @@ def batched(args):
         "seg_unique_u64": lib.pxb_plan_create(ctx.h, C.OP_FLAGS, PX_U64, ctypes.byref(ud), d2.ptr, u2.ptr, o2, nseg2, c2.ptr),
     }
     assert all(plans.values()), plans
+    tiles = {}
+    for k, p in plans.items():
+        blk, it, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib.pxb_plan_tile(p, ctypes.byref(blk), ctypes.byref(it), ctypes.byref(ms)), "plan_tile")
+        if blk.value:
+            tiles[k] = {"threads": blk.value, "vectors_per_thread": it.value, "tuned_ms": round(ms.value, 4)}
     # parity (after the plan warm-up runs): per segment vs float64 reference / numpy
     ctx.sync()
     got = r1.download(np.float32, nseg1)
@@ -334,7 +340,7 @@ def batched(args):
                                    "(seg sort + unique); hipGraph replays", "scale": s},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": per[dom]["gbps"], "peak": HBM_PEAK_GBPS,
                          "unit": "GB/s", "frac": round(per[dom]["gbps"] / HBM_PEAK_GBPS, 4), "traffic": None},
-            "per_primitive": per, "parity": parity}
+            "per_primitive": per, "autotuned_tiles": tiles, "parity": parity}
 
 
 # ------------------------------------------------------------------ served form
@@ -641,7 +647,8 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": r["dtype"], "data": "synthetic", "config": r["config"],
             "roofline": r.get("roofline"), "cpu_baseline": cpu}
-    for k in ("per_primitive", "kernels", "parity", "latency_ms", "req_per_s", "speedup_vs_plain"):
+    for k in ("per_primitive", "kernels", "autotuned_tiles", "parity", "latency_ms", "req_per_s",
+              "speedup_vs_plain"):
         if k in r:
             line[k] = r[k]
     print(json.dumps(line), flush=True)

@@ -121,11 +121,26 @@ int pxb_seg_copy_if(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void*
 
 /* ---- graph plans: capture one batched call into a hipGraph once, replay it many
  * times (same buffers, same offsets).  op: PX_OP_REDUCE, PX_OP_SCAN, PX_OP_SORT,
- * PX_OP_FLAGS (segmented copy_if/unique with `flags`). */
+ * PX_OP_FLAGS (segmented copy_if/unique with `flags`).  For REDUCE, SCAN and FLAGS the
+ * tile shape is autotuned at creation: each candidate shape is timed on this batch and
+ * the fastest is captured (PARALLAX_AUTOTUNE=0 keeps the size-based default;
+ * PARALLAX_TUNE_CACHE=<file> keeps choices across processes).  Creation runs the call
+ * several times: d_out and d_counts are overwritten; an in-place call (d_in == d_out) is
+ * timed into a scratch output, so its input is intact when the plan is returned. */
 typedef struct pxb_plan* pxb_plan_t;
 pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flags,
                            const void* d_in, void* d_out, const uint64_t* h_offsets, size_t nseg,
                            uint64_t* d_counts);
+/* Force the tile shape (threads per block, 16-byte vectors per thread) of one op and
+ * dtype for the calls that follow: PX_OP_SCAN and PX_OP_FLAGS (plain and batched; shapes
+ * 256x8, 512x8, 1024x4, 1024x8) and PX_OP_REDUCE (batched; 256x4, 256x8, 256x16).
+ * block = items = 0 restores the automatic choice.  Not synchronised with calls in
+ * flight on other threads.  Plans created while a shape is forced skip the autotuner. */
+int pxr_tile_override(int op, int dtype, int block, int items);
+/* Tile shape of the captured call (threads per block, 16-byte vectors per thread; 0 for
+ * sort, whose shape is fixed) and the autotuner's best timed duration in ms (0 when the
+ * default was kept). */
+int pxb_plan_tile(pxb_plan_t plan, int* block, int* items, double* tuned_ms);
 int pxb_plan_launch(pxb_plan_t plan);
 void pxb_plan_destroy(pxb_plan_t plan);
 

@@ -3,10 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
 
+#include "autotune.hpp"
 #include "engine.hpp"
 #include "parallax/px_device.h"
 
@@ -21,6 +23,8 @@ struct pxb_plan {
   std::unique_ptr<StreamContext> c;  // the plan owns its scratch and tables
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  pxk::TileCfg tile{0, 0};  // tile shape of the captured call (0: fixed-shape op)
+  double tuned_ms = 0.0;
 };
 
 namespace {
@@ -279,16 +283,21 @@ int pxb_seg_copy_if(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void*
 // ---- graph plans
 namespace {
 int plan_call(StreamContext& c, int op, int dtype, const px_desc* flags, const void* d_in,
-              void* d_out, const parallax::SegSpec& sp, uint64_t* d_counts) {
+              void* d_out, const parallax::SegSpec& sp, uint64_t* d_counts, pxk::TileCfg tile) {
   switch (op) {
-    case PX_OP_REDUCE: return parallax::run_seg_reduce(c, dtype, d_in, sp, d_out);
-    case PX_OP_SCAN: return parallax::run_seg_scan(c, dtype, d_in, d_out, sp);
+    case PX_OP_REDUCE: return parallax::run_seg_reduce(c, dtype, d_in, sp, d_out, tile);
+    case PX_OP_SCAN: return parallax::run_seg_scan(c, dtype, d_in, d_out, sp, tile);
     case PX_OP_SORT: return parallax::run_seg_sort(c, dtype, d_out, sp);
     case PX_OP_FLAGS:
       if (!flags) return PX_EARG;
-      return parallax::run_seg_compact(c, *flags, d_in, d_out, sp, d_counts);
+      return parallax::run_seg_compact(c, *flags, d_in, d_out, sp, d_counts, tile);
     default: return PX_EUNSUPPORTED;
   }
+}
+
+bool autotune_enabled() {
+  const char* e = std::getenv("PARALLAX_AUTOTUNE");
+  return !(e && e[0] == '0');
 }
 }  // namespace
 
@@ -303,8 +312,38 @@ pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flag
     return nullptr;
   }
   const parallax::SegSpec sp = contiguous(h_offsets, nseg);
+  int rc = PX_EOK;
+  // 0) tile shape: timed on this batch (scan / compaction / reduce).  An in-place call
+  //    is timed into a scratch output so its input survives.
+  if (op == PX_OP_REDUCE || op == PX_OP_SCAN || op == PX_OP_FLAGS) {
+    const int dt = (op == PX_OP_FLAGS && flags) ? static_cast<int>(flags->dtype) : dtype;
+    p->tile = parallax::seg_tile_default(op, dt, sp);
+    if (autotune_enabled() && nseg > 0 && dt >= 0 && dt < PX_NUM_DTYPES &&
+        pxk::tile_override(op, dt).block == 0) {
+      void* out = d_out;
+      void* tmp = nullptr;
+      if (op != PX_OP_REDUCE && d_in == d_out) {
+        const size_t bytes = static_cast<size_t>(h_offsets[nseg]) * pxk::dtype_size(dt) + 16;
+        if (hipMalloc(&tmp, bytes) != hipSuccess) {
+          (void)hipGetLastError();
+          tmp = nullptr;
+        }
+        out = tmp;
+      }
+      if (out) {
+        const parallax::TuneChoice ch = parallax::autotune_tile(
+            *p->c, op, dt, sp, [&](pxk::TileCfg t) {
+              return plan_call(*p->c, op, dtype, flags, d_in, out, sp, d_counts, t);
+            });
+        p->tile = ch.tile;
+        p->tuned_ms = ch.ms;
+        (void)hipStreamSynchronize(p->c->stream());
+      }
+      if (tmp) (void)hipFree(tmp);
+    }
+  }
   // 1) warm-up run: uploads the tables and sizes the scratch
-  int rc = plan_call(*p->c, op, dtype, flags, d_in, d_out, sp, d_counts);
+  rc = plan_call(*p->c, op, dtype, flags, d_in, d_out, sp, d_counts, p->tile);
   if (rc == PX_EOK) rc = hip_rc(hipStreamSynchronize(p->c->stream()), "plan warm-up");
   // 2) capture the same call with uploads frozen
   if (rc == PX_EOK) {
@@ -312,7 +351,7 @@ pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flag
     rc = hip_rc(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc == PX_EOK) {
       p->c->freeze(true);
-      const int crc = plan_call(*p->c, op, dtype, flags, d_in, d_out, sp, d_counts);
+      const int crc = plan_call(*p->c, op, dtype, flags, d_in, d_out, sp, d_counts, p->tile);
       p->c->freeze(false);
       const int erc = hip_rc(hipStreamEndCapture(s, &p->graph), "end capture");
       rc = crc ? crc : erc;
@@ -325,6 +364,22 @@ pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flag
     return nullptr;
   }
   return p;
+}
+
+int pxr_tile_override(int op, int dtype, int block, int items) {
+  if (!pxk::set_tile_override(op, dtype, pxk::TileCfg{block, items})) {
+    parallax::set_last_error(PX_EARG, "pxr_tile_override: unsupported op, dtype or tile shape");
+    return PX_EARG;
+  }
+  return PX_EOK;
+}
+
+int pxb_plan_tile(pxb_plan_t plan, int* block, int* items, double* tuned_ms) {
+  if (!plan) return PX_EARG;
+  if (block) *block = plan->tile.block;
+  if (items) *items = plan->tile.k;
+  if (tuned_ms) *tuned_ms = plan->tuned_ms;
+  return PX_EOK;
 }
 
 int pxb_plan_launch(pxb_plan_t plan) {

@@ -187,6 +187,13 @@ int zero_dev(StreamContext& c, void* p, size_t bytes) {
   return dev_err(pxk::fill_u32(static_cast<uint32_t*>(p), bytes / 4, 0u, c.stream()), "fill");
 }
 
+// Tile of a plain scan / compaction: the forced shape, else the per-type default.
+pxk::TileCfg plain_tile(int op, int dtype) {
+  const pxk::TileCfg forced = pxk::tile_override(op, dtype);
+  if (forced.block) return forced;
+  return op == PX_OP_SCAN ? pxk::tune().scan[dtype] : pxk::tune().compact[dtype];
+}
+
 pxk::SegGeom plain_geom(uint64_t n, int dtype, pxk::TileCfg k, const void* in) {
   pxk::SegGeom g{};
   const uint32_t tile = pxk::tile_elems(dtype, k);
@@ -265,7 +272,7 @@ int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n,
   if (n == 0) return PX_OK;
   ++g_stats.scan;
   ++g_stats.launches;
-  const pxk::SegGeom g = plain_geom(n, dtype, pxk::tune().scan[dtype], in);
+  const pxk::SegGeom g = plain_geom(n, dtype, plain_tile(PX_OP_SCAN, dtype), in);
   if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0))
     return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
@@ -286,7 +293,7 @@ int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* ou
   ++g_stats.compact;
   ++g_stats.launches;
   if (n == 0) return zero_dev(c, count_dev, sizeof(uint64_t));
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().compact[flags.dtype], in);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, plain_tile(PX_OP_FLAGS, flags.dtype), in);
   const size_t words = n >= (1ull << 32) ? 2 : 1;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
@@ -304,7 +311,7 @@ int run_partition(StreamContext& c, const px_desc& flags, const void* in, void* 
   if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16)) return PX_ERR_NOMEM;
   int rc = dev_err(pxk::count_if(flags, in, n, num_true_dev, c.scratch(), c.stream()), "count");
   if (rc) return rc;
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, pxk::tune().compact[flags.dtype], in);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, plain_tile(PX_OP_FLAGS, flags.dtype), in);
   const size_t words = n >= (1ull << 32) ? 2 : 1;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
@@ -354,27 +361,60 @@ int run_mismatch(StreamContext& c, int dtype, const void* a, const void* b, uint
   return dev_err(pxk::mismatch(dtype, a, b, n, out_dev, c.scratch(), c.stream()), "mismatch");
 }
 
-int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& sp, void* out_dev) {
+std::vector<pxk::TileCfg> seg_tile_candidates(int op, int dtype) {
+  std::vector<pxk::TileCfg> v;
+  if (op == PX_OP_REDUCE) {
+    for (int k : {4, 8, 16}) v.push_back({256, k});
+  } else {
+    for (pxk::TileCfg t : {pxk::TileCfg{256, 8}, pxk::TileCfg{512, 8}, pxk::TileCfg{1024, 4},
+                           pxk::TileCfg{1024, 8}})
+      if (pxk::tile_cfg_supported(t) && !(dtype == PX_F64 && t.block == 1024 && t.k == 8)) v.push_back(t);
+  }
+  return v;
+}
+
+pxk::TileCfg seg_tile_default(int op, int dtype, const SegSpec& sp) {
+  const pxk::TileCfg forced = pxk::tile_override(op, dtype);
+  if (forced.block) return forced;
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < sp.nseg; ++s) total += sp.hi[s] - sp.lo[s];
+  const uint64_t mean = sp.nseg ? total / sp.nseg : 0;
+  const pxk::TileCfg big = op == PX_OP_REDUCE ? pxk::tune().seg_reduce
+                          : op == PX_OP_SCAN  ? pxk::tune().scan[dtype]
+                                              : pxk::tune().compact[dtype];
+  pxk::TileCfg best{0, 0};
+  for (const pxk::TileCfg& t : seg_tile_candidates(op, dtype)) {
+    if (best.block == 0) best = t;  // smallest candidate first
+    if (pxk::tile_elems(dtype, t) <= mean && pxk::tile_elems(dtype, t) <= pxk::tile_elems(dtype, big)) best = t;
+  }
+  return best;
+}
+
+int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& sp, void* out_dev,
+                   pxk::TileCfg tile) {
   if (!valid_dtype(dtype) || !out_dev || !in || !valid_spec(sp)) return PX_ERR_ARG;
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, dtype, pxk::tune().seg_reduce, in, g, tt);
+  if (tile.block == 0) tile = seg_tile_default(PX_OP_REDUCE, dtype, sp);
+  int rc = seg_geom(c, sp, dtype, tile, in, g, tt);
   if (rc) return rc;
   if (!c.reserve(0, static_cast<size_t>(g.ntiles) * 16)) return PX_ERR_NOMEM;
   return dev_err(pxk::seg_reduce(dtype, in, g, out_dev, c.scratch(), c.stream()), "seg_reduce");
 }
 
-int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const SegSpec& sp) {
+int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const SegSpec& sp,
+                 pxk::TileCfg tile) {
   if (!valid_dtype(dtype) || !in || !out || !valid_spec(sp)) return PX_ERR_ARG;
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, dtype, pxk::tune().scan[dtype], in, g, tt);
+  if (tile.block == 0) tile = seg_tile_default(PX_OP_SCAN, dtype, sp);
+  int rc = seg_geom(c, sp, dtype, tile, in, g, tt);
   if (rc) return rc;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0))
     return PX_ERR_NOMEM;
@@ -383,14 +423,15 @@ int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const S
 }
 
 int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void* out,
-                    const SegSpec& sp, uint64_t* counts_dev) {
+                    const SegSpec& sp, uint64_t* counts_dev, pxk::TileCfg tile) {
   if (!valid_dtype(flags.dtype) || !in || !out || !counts_dev || !valid_spec(sp)) return PX_ERR_ARG;
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
   pxk::SegGeom g;
   TileTables tt;
-  int rc = seg_geom(c, sp, flags.dtype, pxk::tune().compact[flags.dtype], in, g, tt);
+  if (tile.block == 0) tile = seg_tile_default(PX_OP_FLAGS, flags.dtype, sp);
+  int rc = seg_geom(c, sp, flags.dtype, tile, in, g, tt);
   if (rc) return rc;
   if (!c.reserve(static_cast<size_t>(g.ntiles), 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;

@@ -123,11 +123,19 @@ int run_mismatch(StreamContext& c, int dtype, const void* a, const void* b, uint
                  uint64_t* out_dev);
 
 // segmented (batched) runners
-int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& sp, void* out_dev);
-int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const SegSpec& sp);
+// Batched runners.  `tile` picks the tile shape ({0, 0}: seg_tile_default()).
+int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& sp, void* out_dev,
+                   pxk::TileCfg tile = {0, 0});
+int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const SegSpec& sp,
+                 pxk::TileCfg tile = {0, 0});
 int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp);
 int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void* out,
-                    const SegSpec& sp, uint64_t* counts_dev);
+                    const SegSpec& sp, uint64_t* counts_dev, pxk::TileCfg tile = {0, 0});
+// Default tile of a batched call: the largest candidate no longer than the mean
+// segment (small segments waste less of a small tile).
+pxk::TileCfg seg_tile_default(int op, int dtype, const SegSpec& sp);
+// Tile shapes the autotuner tries for `op` (REDUCE, SCAN, FLAGS).
+std::vector<pxk::TileCfg> seg_tile_candidates(int op, int dtype);
 
 // Counters of device work actually launched (tests check the GPU path ran).
 struct EngineStats {

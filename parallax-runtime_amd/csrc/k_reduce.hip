@@ -221,7 +221,15 @@ hipError_t seg_reduce_t(const void* in, const SegGeom& g, void* out, const Scrat
   if (grid == 0) return hipSuccess;
   A* part = reinterpret_cast<A*>(s.partials);
   T* o = reinterpret_cast<T*>(out);
-  k_seg_reduce<T, kSegReduceK><<<grid, 256, 0, stream>>>(in_al, shift, g, part, o);
+  if (g.tile_block != 256) return hipErrorInvalidValue;
+  if (g.tile_k == 4)
+    k_seg_reduce<T, 4><<<grid, 256, 0, stream>>>(in_al, shift, g, part, o);
+  else if (g.tile_k == 8)
+    k_seg_reduce<T, 8><<<grid, 256, 0, stream>>>(in_al, shift, g, part, o);
+  else if (g.tile_k == 16)
+    k_seg_reduce<T, 16><<<grid, 256, 0, stream>>>(in_al, shift, g, part, o);
+  else
+    return hipErrorInvalidValue;
   if (g.ntiles > g.nseg)  // some segment spans more than one tile
     k_seg_combine<T><<<(g.nseg + 3) / 4, 256, 0, stream>>>(g, part, o);
   return hipGetLastError();

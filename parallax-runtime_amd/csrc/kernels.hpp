@@ -53,9 +53,12 @@ struct Tune {
 };
 Tune& tune();
 bool tile_cfg_supported(TileCfg c);
+// Forced tile shape of (op, dtype) for plain and batched calls (op: PX_OP_REDUCE
+// (batched only), PX_OP_SCAN, PX_OP_FLAGS); {0, 0} when none is set.
+TileCfg tile_override(int op, int dtype);
+bool set_tile_override(int op, int dtype, TileCfg c);  // {0, 0} clears
 // Persistent grid of a look-back launch: every tile, at most the resident blocks.
 uint32_t lookback_grid(uint32_t ntiles, uint32_t max_blocks, int block);
-constexpr int kSegReduceK = 4;
 
 // Scratch of one stream context.  All of it is device memory; ctl is zeroed once with
 // epoch = 1 and is left clean by every kernel.

@@ -12,6 +12,30 @@ bool tile_cfg_supported(TileCfg c) {
          (c.block == 1024 && (c.k == 4 || c.k == 8));
 }
 
+namespace {
+TileCfg g_override[3][PX_NUM_DTYPES] = {};
+int override_slot(int op) {
+  return op == PX_OP_REDUCE ? 0 : op == PX_OP_SCAN ? 1 : op == PX_OP_FLAGS ? 2 : -1;
+}
+}  // namespace
+
+TileCfg tile_override(int op, int dtype) {
+  const int i = override_slot(op);
+  if (i < 0 || dtype < 0 || dtype >= PX_NUM_DTYPES) return TileCfg{0, 0};
+  return g_override[i][dtype];
+}
+
+bool set_tile_override(int op, int dtype, TileCfg c) {
+  const int i = override_slot(op);
+  if (i < 0 || dtype < 0 || dtype >= PX_NUM_DTYPES) return false;
+  const bool clear = c.block == 0 && c.k == 0;
+  const bool ok = clear || (op == PX_OP_REDUCE ? (c.block == 256 && (c.k == 4 || c.k == 8 || c.k == 16))
+                                               : tile_cfg_supported(c));
+  if (!ok) return false;
+  g_override[i][dtype] = c;
+  return true;
+}
+
 uint32_t lookback_grid(uint32_t ntiles, uint32_t max_blocks, int block) {
   const uint32_t cus = max_blocks / 8;
   const int bpc = tune().bpc > 0 ? tune().bpc : (1024 / block > 0 ? 1024 / block : 1);

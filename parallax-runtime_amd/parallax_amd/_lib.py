@@ -94,6 +94,8 @@ _proto("pxb_seg_inclusive_scan", _i, _vp, _i, _vp, _vp, _u64p, _sz)
 _proto("pxb_seg_sort", _i, _vp, _i, _vp, _u64p, _sz)
 _proto("pxb_seg_copy_if", _i, _vp, _dp, _vp, _vp, _u64p, _sz, _vp)
 _proto("pxb_plan_create", _vp, _vp, _i, _i, _dp, _vp, _vp, _u64p, _sz, _vp)
+_proto("pxb_plan_tile", _i, _vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double))
+_proto("pxr_tile_override", _i, _i, _i, _i, _i)
 _proto("pxb_plan_launch", _i, _vp)
 _proto("pxb_plan_destroy", None, _vp)
 

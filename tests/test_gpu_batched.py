@@ -147,6 +147,139 @@ def test_alone_equals_in_the_mix(gpu_ctx, dt):
             assert alone.tobytes() == ref.tobytes()
 
 
+SCAN_TILES = [(256, 8), (512, 8), (1024, 4), (1024, 8)]
+REDUCE_TILES = [(256, 4), (256, 8), (256, 16)]
+
+
+@pytest.fixture
+def no_tile_override():
+    yield
+    for op in (C.OP_REDUCE, C.OP_SCAN, C.OP_FLAGS):
+        for dt in ALL:
+            lib.pxr_tile_override(op, dt, 0, 0)
+
+
+def check_seg_scan(a, off, got, dt):
+    if dt in FLOATS:
+        for s in range(off.size - 1):
+            seg = a[off[s]:off[s + 1]]
+            if seg.size:
+                val, ab = pxo.scan_exact(seg)
+                err = np.abs(got[off[s]:off[s + 1]].astype(np.float64) - val)
+                assert np.all(err <= bound(dt, seg.size) * ab + 1e-300), s
+    else:
+        assert np.array_equal(got, pxo.seg_inclusive_scan(a, off))
+
+
+@pytest.mark.parametrize("tile", SCAN_TILES)
+@pytest.mark.parametrize("dt", [PX_U32, PX_I64, PX_F32, PX_F64])
+def test_every_tile_shape(gpu_ctx, no_tile_override, dt, tile):
+    """Every compiled look-back tile shape, forced through pxr_tile_override, gives the
+    oracle's results: batched scan and copy_if, plain scan and copy_if on a pointer one
+    element past a 16-byte boundary, and every batched reduce shape."""
+    check(lib.pxr_tile_override(C.OP_SCAN, dt, *tile), "override scan")
+    check(lib.pxr_tile_override(C.OP_FLAGS, dt, *tile), "override flags")
+    a, off = seg_input(dt, 200 + dt + tile[0] + tile[1], nseg=90, hi=150_000)
+    nseg = off.size - 1
+    d = DeviceBuffer.from_array(a)
+    o = DeviceBuffer(a.nbytes + 16)
+    cnt = DeviceBuffer(8 * nseg)
+    check(lib.pxb_seg_inclusive_scan(gpu_ctx.h, dt, d.ptr, o.ptr, u64p(off), nseg), "seg_scan")
+    gpu_ctx.sync()
+    check_seg_scan(a, off, o.download(a.dtype, a.size), dt)
+    fl = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=0.0 if dt in FLOATS else 500)
+    check(lib.pxb_seg_copy_if(gpu_ctx.h, ctypes.byref(fl), d.ptr, o.ptr, u64p(off), nseg, cnt.ptr), "seg_copy_if")
+    gpu_ctx.sync()
+    got = o.download(a.dtype, a.size)
+    counts = cnt.download(np.uint64, nseg)
+    for s in range(nseg):
+        ref = pxo.copy_if(fl, a[off[s]:off[s + 1]])
+        assert counts[s] == ref.size and got[off[s]:off[s] + ref.size].tobytes() == ref.tobytes()
+    # plain calls, misaligned by one element, several tiles long
+    n = 300_001
+    es = a.itemsize
+    b = a[:n] if a.size >= n else np.resize(a, n)
+    pd = DeviceBuffer(b.nbytes + 64)
+    pd.upload(b, offset=es)
+    po = DeviceBuffer(b.nbytes + 64)
+    check(lib.pxd_inclusive_scan(gpu_ctx.h, dt, pd.ptr + es, po.ptr + es, n), "scan")
+    cn = DeviceBuffer(8)
+    gpu_ctx.sync()
+    got = po.download(b.dtype, n, offset=es)
+    one = np.array([0, n], dtype=np.uint64)
+    check_seg_scan(b, one, got, dt)
+    check(lib.pxd_copy_if(gpu_ctx.h, ctypes.byref(fl), pd.ptr + es, po.ptr, n, cn.ptr), "copy_if")
+    gpu_ctx.sync()
+    ref = pxo.copy_if(fl, b)
+    k = int(cn.download(np.uint64, 1)[0])
+    assert k == ref.size and po.download(b.dtype, k).tobytes() == ref.tobytes()
+    # batched reduce, every shape
+    r = DeviceBuffer(8 * nseg)
+    for rt in REDUCE_TILES:
+        check(lib.pxr_tile_override(C.OP_REDUCE, dt, *rt), "override reduce")
+        check(lib.pxb_seg_reduce(gpu_ctx.h, dt, d.ptr, u64p(off), nseg, r.ptr), "seg_reduce")
+        gpu_ctx.sync()
+        got = r.download(a.dtype, nseg)
+        if dt in FLOATS:
+            hi, ab = pxo.seg_sum_exact(a, off)
+            lens = np.diff(off.astype(np.int64))
+            for s in range(nseg):
+                assert abs(float(got[s]) - hi[s]) <= bound(dt, lens[s]) * ab[s] + 1e-300, (rt, s)
+        else:
+            assert np.array_equal(got, pxo.seg_reduce(a, off)), rt
+
+
+def test_tile_override_rejects_unsupported(no_tile_override):
+    assert lib.pxr_tile_override(C.OP_SCAN, PX_U32, 128, 8) != 0
+    assert lib.pxr_tile_override(C.OP_REDUCE, PX_U32, 512, 4) != 0
+    assert lib.pxr_tile_override(C.OP_SORT, PX_U32, 256, 8) != 0
+    assert lib.pxr_tile_override(C.OP_SCAN, 99, 256, 8) != 0
+
+
+@pytest.mark.parametrize("op,dt", [(C.OP_SCAN, PX_U32), (C.OP_SCAN, PX_F64), (C.OP_FLAGS, PX_I32),
+                                   (C.OP_REDUCE, PX_F32)])
+def test_plan_autotune(gpu_ctx, no_tile_override, op, dt):
+    """The plan's autotuner picks one of the candidate shapes, reports a timing, and the
+    captured call gives the oracle's result.  An in-place scan plan leaves its input
+    intact through tuning."""
+    a, off = seg_input(dt, 300 + op + dt, nseg=200, hi=120_000)
+    nseg = off.size - 1
+    d = DeviceBuffer.from_array(a)
+    o = DeviceBuffer(max(a.nbytes, 8 * nseg) + 16)
+    cnt = DeviceBuffer(8 * nseg)
+    fl = make_desc(C.OP_FLAGS, dt, fn=C.PRED_LT, c0=300)
+    inplace = op == C.OP_SCAN
+    ctx = Context()
+    plan = lib.pxb_plan_create(ctx.h, op, dt, ctypes.byref(fl), d.ptr, d.ptr if inplace else o.ptr,
+                               u64p(off), nseg, cnt.ptr)
+    assert plan, "plan creation failed"
+    blk, k, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    check(lib.pxb_plan_tile(plan, ctypes.byref(blk), ctypes.byref(k), ctypes.byref(ms)), "plan_tile")
+    shapes = REDUCE_TILES if op == C.OP_REDUCE else SCAN_TILES
+    assert (blk.value, k.value) in shapes
+    assert ms.value > 0.0
+    if inplace:
+        ctx.sync()
+        assert d.download(a.dtype, a.size).tobytes() == a.tobytes(), "input changed by tuning"
+    check(lib.pxb_plan_launch(plan), "launch")
+    ctx.sync()
+    if op == C.OP_SCAN:
+        check_seg_scan(a, off, d.download(a.dtype, a.size), dt)
+    elif op == C.OP_FLAGS:
+        counts = cnt.download(np.uint64, nseg)
+        got = o.download(a.dtype, a.size)
+        for s in range(nseg):
+            ref = pxo.copy_if(fl, a[off[s]:off[s + 1]])
+            assert counts[s] == ref.size and np.array_equal(got[off[s]:off[s] + ref.size], ref)
+    else:
+        got = o.download(a.dtype, nseg)
+        hi, ab = pxo.seg_sum_exact(a, off)
+        lens = np.diff(off.astype(np.int64))
+        assert all(abs(float(got[s]) - hi[s]) <= bound(dt, lens[s]) * ab[s] + 1e-300 for s in range(nseg))
+    lib.pxb_plan_destroy(plan)
+    ctx.close()
+
+
 def test_graph_plan_replays(gpu_ctx):
     """A captured plan replayed several times, with other eager work between replays,
     gives the eager result every time (no memset nodes; self-resetting scratch)."""
