@@ -313,38 +313,39 @@ pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flag
   }
   const parallax::SegSpec sp = contiguous(h_offsets, nseg);
   int rc = PX_EOK;
-  // 0) tile shape: timed on this batch (scan / compaction / reduce).  An in-place call
-  //    is timed into a scratch output so its input survives.
+  // An in-place scan / compaction runs its tuning and warm-up calls into a scratch
+  // output, so the caller's input is intact when the plan is returned.
+  const int dt = (op == PX_OP_FLAGS && flags) ? static_cast<int>(flags->dtype) : dtype;
+  void* warm_out = d_out;
+  void* tmp = nullptr;
+  if ((op == PX_OP_SCAN || op == PX_OP_FLAGS) && d_in == d_out && nseg > 0 && dt >= 0 &&
+      dt < PX_NUM_DTYPES) {
+    const size_t bytes = static_cast<size_t>(h_offsets[nseg]) * pxk::dtype_size(dt) + 16;
+    if (hipMalloc(&tmp, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      parallax::set_last_error(PX_ENOMEM, "plan scratch output");
+      pxb_plan_destroy(p);
+      return nullptr;
+    }
+    warm_out = tmp;
+  }
+  // 0) tile shape: timed on this batch (scan / compaction / reduce)
   if (op == PX_OP_REDUCE || op == PX_OP_SCAN || op == PX_OP_FLAGS) {
-    const int dt = (op == PX_OP_FLAGS && flags) ? static_cast<int>(flags->dtype) : dtype;
     p->tile = parallax::seg_tile_default(op, dt, sp);
     if (autotune_enabled() && nseg > 0 && dt >= 0 && dt < PX_NUM_DTYPES &&
         pxk::tile_override(op, dt).block == 0) {
-      void* out = d_out;
-      void* tmp = nullptr;
-      if (op != PX_OP_REDUCE && d_in == d_out) {
-        const size_t bytes = static_cast<size_t>(h_offsets[nseg]) * pxk::dtype_size(dt) + 16;
-        if (hipMalloc(&tmp, bytes) != hipSuccess) {
-          (void)hipGetLastError();
-          tmp = nullptr;
-        }
-        out = tmp;
-      }
-      if (out) {
-        const parallax::TuneChoice ch = parallax::autotune_tile(
-            *p->c, op, dt, sp, [&](pxk::TileCfg t) {
-              return plan_call(*p->c, op, dtype, flags, d_in, out, sp, d_counts, t);
-            });
-        p->tile = ch.tile;
-        p->tuned_ms = ch.ms;
-        (void)hipStreamSynchronize(p->c->stream());
-      }
-      if (tmp) (void)hipFree(tmp);
+      const parallax::TuneChoice ch = parallax::autotune_tile(
+          *p->c, op, dt, sp, [&](pxk::TileCfg t) {
+            return plan_call(*p->c, op, dtype, flags, d_in, warm_out, sp, d_counts, t);
+          });
+      p->tile = ch.tile;
+      p->tuned_ms = ch.ms;
     }
   }
   // 1) warm-up run: uploads the tables and sizes the scratch
-  rc = plan_call(*p->c, op, dtype, flags, d_in, d_out, sp, d_counts, p->tile);
+  rc = plan_call(*p->c, op, dtype, flags, d_in, warm_out, sp, d_counts, p->tile);
   if (rc == PX_EOK) rc = hip_rc(hipStreamSynchronize(p->c->stream()), "plan warm-up");
+  if (tmp) (void)hipFree(tmp);
   // 2) capture the same call with uploads frozen
   if (rc == PX_EOK) {
     hipStream_t s = p->c->stream();
