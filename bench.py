@@ -108,6 +108,7 @@ def plain_suite(args):
     t0 = time.time()
     step(nsteps - 1)
     parity = verify(X, D, sort_u32[nsteps - 1], sort_f64[nsteps - 1], kept_box[0])
+    parity["c1_reduce_2p20_f32"] = verify_c1(lib, load, C, PX_F32)
     log(f"[bench] parity verified in {time.time() - t0:.1f}s: {parity}")
 
     for i in range(args.warmup):
@@ -233,6 +234,21 @@ def verify(X, D, su, sf, kept):
     assert kept == ref.size and np.array_equal(D["cif_out"].download(np.int32, kept), ref), "copy_if"
     out["bit_exact"] = ["reduce_i64", "scans_u32", "sort_u32", "sort_f64", "copy_if_i32"]
     return out
+
+
+def verify_c1(lib, load, C, PX_F32):
+    """C1: sum of 2^20 float32 uniform [0,1) (seed 0) on the GPU through the C ABI, with
+    its error against the FP64 sum and the stated bound."""
+    import math
+    from parallax_amd import datagen, make_desc
+    a = datagen.uniform01_f32(0, 1 << 20)
+    r = ctypes.c_float()
+    lib.parallax_reduce(load(make_desc(C.OP_REDUCE, PX_F32)), a.ctypes.data, a.size, 4, ctypes.byref(r))
+    exact = math.fsum(a.astype(np.float64))
+    bnd = 2 * math.ceil(math.log2(a.size)) * float(np.finfo(np.float32).eps) * exact
+    err = abs(float(r.value) - exact)
+    assert err <= bnd, ("C1", err, bnd)
+    return {"gpu_sum": float(r.value), "fp64_sum": exact, "abs_err": err, "bound": bnd, "err_over_bound": err / bnd}
 
 
 # ------------------------------------------------------------------ batched forms
@@ -519,7 +535,7 @@ def copy_probe():
 
 
 # ------------------------------------------------------------------ CPU baseline
-CPU_SAMPLE = {"n_stream": 1 << 26, "n_sort_u32": 1 << 23, "n_sort_f64": 1 << 22}
+CPU_SAMPLE = {"n_stream": 1 << 28, "n_sort_u32": 1 << 26, "n_sort_f64": 1 << 25}
 
 
 def cpu_baseline_child():
@@ -567,7 +583,7 @@ def cpu_baseline_child():
     total_bytes = 0
     for name, bytes_, fn in calls:
         reps = []
-        for _ in range(3):
+        for _ in range(1 if name.startswith("sort") else 3):
             if name == "sort_u32":
                 w = su.copy()
                 t = time.perf_counter()
@@ -582,9 +598,17 @@ def cpu_baseline_child():
             reps.append(time.perf_counter() - t)
             if name == "copy_if_i32":
                 bytes_ = ns * 4 + int(r) * 4
-        times[name] = sorted(reps)[1]
+        times[name] = sorted(reps)[len(reps) // 2]
         total_bytes += bytes_
     total = sum(times.values())
+    # C1: std::reduce over 2^20 f32 uniform [0,1) (seed 0) through the fallback, against FP64
+    import math
+    c1 = datagen.uniform01_f32(0, 1 << 20)
+    fb = float(f("ref_reduce_f32", ctypes.c_float, vp, sz)(c1.ctypes.data, c1.size))
+    exact = math.fsum(c1.astype(np.float64))
+    bnd = 2 * math.ceil(math.log2(c1.size)) * float(np.finfo(np.float32).eps) * exact
+    c1r = {"fallback_sum": fb, "fp64_sum": exact, "abs_err": abs(fb - exact), "bound": bnd,
+           "err_over_bound": abs(fb - exact) / bnd}
     cpu = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -595,10 +619,11 @@ def cpu_baseline_child():
         pass
     print(json.dumps({"value": round(total_bytes / total / 1e9, 3), "unit": "GB/s", "cores": 1,
                       "kind": kind,
-                      "sample": (f"reference CPU fallback (stdpar.hpp host loops, g++ -O3, 1 core, median of 3) "
-                                 f"on the same suite at reduced sizes: streams 2^26, sort u32 2^23, "
-                                 f"sort f64 2^22; host {cpu}"),
-                      "seconds": {k: round(v, 4) for k, v in times.items()}}))
+                      "sample": (f"reference CPU fallback (stdpar.hpp host loops, g++ -O3, 1 core; median of 3 "
+                                 f"for the streaming primitives, one run per sort) "
+                                 f"on the full suite: streams 2^28, sort u32 2^26, sort f64 2^25, same "
+                                 f"generators; host {cpu}"),
+                      "seconds": {k: round(v, 4) for k, v in times.items()}, "c1_reduce_2p20_f32": c1r}))
 
 
 def cpu_baseline():
