@@ -313,7 +313,7 @@ __device__ __forceinline__ uint32_t peek(const uint64_t* status, uint64_t tile, 
 // waits on nothing beyond that.  Returns the exclusive prefix of `tile` in every lane.
 // Every predecessor has already taken its ticket (tickets are handed out in order), so
 // every wait ends.
-constexpr int kLookbackR = 4;
+constexpr int kLookbackR = 1;
 
 template <class A>
 __device__ __forceinline__ A lookback(const uint64_t* status, uint64_t tile, int64_t first,
