@@ -173,6 +173,58 @@ def plain_suite(args):
     ms_step = wall * 1e3 / args.steps
     value = step_bytes / (ms_step * 1e-3) / 1e9
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    # The drop-in path with host operands (the reference's interface): every call stages
+    # its operand over PCIe and copies the result back.  Reported beside, never `value`.
+    host = {}
+    hx = {"red_f32": X["reduce_f32"], "red_i64": X["reduce_i64"], "scan": X["scan_u32"].copy(),
+          "sort_u32": X["sort_u32"].copy(), "sort_f64": X["sort_f64"].copy(), "cif": X["copy_if_i32"]}
+    hres = np.zeros(2, dtype=np.int64)
+    hout = np.empty_like(X["copy_if_i32"])
+    host_calls = [
+        ("reduce_f32", n28 * 4, lambda: lib.parallax_reduce(K["red_f32"], hx["red_f32"].ctypes.data, n28, 4, hres.ctypes.data)),
+        ("reduce_i64", n28 * 8, lambda: lib.parallax_reduce(K["red_i64"], hx["red_i64"].ctypes.data, n28, 8, hres.ctypes.data)),
+        ("inclusive_scan_u32", 2 * n28 * 4, lambda: lib.parallax_scan(K["scan"], K["add"], hx["scan"].ctypes.data, n28, 4)),
+        ("sort_u32", 2 * n26 * 4, lambda: lib.parallax_sort(K["sort_u32"], hx["sort_u32"].ctypes.data, n26, 4)),
+        ("sort_f64", 2 * n25 * 8, lambda: lib.parallax_sort(K["sort_f64"], hx["sort_f64"].ctypes.data, n25, 8)),
+        ("copy_if_i32", n28 * 4 + kept * 4, lambda: lib.parallax_copy_if(K["flags"], K["scan_i32"], K["add_i32"], K["scatter"],
+                                                                        hx["cif"].ctypes.data, hout.ctypes.data, n28, 4, 0)),
+    ]
+    for name, bytes_, fn in host_calls:
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        host[name] = {"ms": round(dt * 1e3, 3), "gbps": round(bytes_ / dt / 1e9, 1)}
+    log(f"[bench] C ABI with host operands (PCIe-inclusive): {host}")
+    # Fused map-reduce (the transform_reduce / count_if funnels' device pass), one read of
+    # the input: squared sum of the 2^28 f32 input in f64, count of even values of the
+    # 2^28 i32 input.  Device time per call from the launcher's events.
+    fused = {}
+    sq = load(make_desc(C.OP_TRANSFORM, PX_F32, aux=PX_F64, fn=C.MAP_SQUARE))
+    red64 = load(make_desc(C.OP_REDUCE, PX_F64))
+    ev = load(make_desc(C.OP_TRANSFORM, PX_I32, aux=PX_I32, fn=C.MAP_PRED, fn2=C.PRED_MOD_EQ, c0=2, c1=0))
+    red32 = load(make_desc(C.OP_REDUCE, PX_I32))
+    fres = np.zeros(2, dtype=np.float64)
+    cres = np.zeros(2, dtype=np.int32)
+    lib.pxr_prof_enable(1)
+    for _ in range(args.warmup + args.steps):
+        lib.parallax_transform_reduce(sq, red64, D["red_f32"].ptr, n28, 4, 8, fres.ctypes.data)
+        lib.parallax_transform_reduce(ev, red32, D["cif_in"].ptr, n28, 4, 4, cres.ctypes.data)
+    for key, name, dt_in, nb in (("transform_reduce:f32", "square_sum_f32_to_f64", "f32", n28 * 4),
+                                 ("transform_reduce:i32", "count_if_even_i32", "i32", n28 * 4)):
+        tot, calls = prof(key)
+        ms = tot / max(calls, 1)
+        fused[name] = {"ms": round(ms, 4), "alg_bytes": nb, "gbps": round(nb / (ms * 1e-3) / 1e9, 1),
+                       "pct_hbm_peak": round(100 * nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1)}
+    lib.pxr_prof_enable(0)
+    import math
+    a32 = X["reduce_f32"].astype(np.float64)
+    exact = float(np.dot(a32, a32))
+    bnd = 2 * math.ceil(math.log2(n28)) * float(np.finfo(np.float64).eps) * exact
+    assert abs(fres[0] - exact) <= bnd, ("square sum", fres[0], exact)
+    even = int(np.count_nonzero((X["copy_if_i32"] & 1) == 0))
+    assert int(cres[0]) == even, ("count_if", int(cres[0]), even)
+    fused["parity"] = {"square_sum_err_over_bound": abs(fres[0] - exact) / bnd, "count_if": "exact"}
+    log(f"[bench] fused map-reduce: {fused}")
     ach = kernels[dom]["gbps"]
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
@@ -187,6 +239,7 @@ def plain_suite(args):
                                "sort 2^26 u32 + 2^25 f64, copy_if 2^28 i32 (x%2==0), seed 1",
                    "scale": scale},
         "roofline": roof, "per_primitive": per, "kernels": kernels, "parity": parity,
+        "c_abi_host_operands": host, "fused_map_reduce": fused,
     }
 
 
@@ -640,8 +693,8 @@ def cpu_baseline():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain", choices=["plain", "batched", "served"])
     ap.add_argument("--scale", type=float, default=1.0, help="size multiplier (1.0 = the configuration)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -672,8 +725,8 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": r["dtype"], "data": "synthetic", "config": r["config"],
             "roofline": r.get("roofline"), "cpu_baseline": cpu}
-    for k in ("per_primitive", "kernels", "autotuned_tiles", "parity", "latency_ms", "req_per_s",
-              "speedup_vs_plain"):
+    for k in ("per_primitive", "kernels", "autotuned_tiles", "parity", "c_abi_host_operands",
+              "fused_map_reduce", "latency_ms", "req_per_s", "speedup_vs_plain"):
         if k in r:
             line[k] = r[k]
     print(json.dumps(line), flush=True)

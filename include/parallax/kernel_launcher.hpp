@@ -51,6 +51,13 @@ class KernelLauncher {
                             void* captures, size_t capture_size, size_t elem_size = sizeof(float));
   bool launch_reduce(const std::string& kernel_name, void* data, size_t count, size_t elem_size,
                      void* out_result);
+  // Σ map(x) in one pass (transform_reduce / count_if funnels): `map_kernel` is a
+  // TRANSFORM descriptor (in dtype, out dtype in aux), `reduce_kernel` a REDUCE of the
+  // out dtype.  Maps the fused kernel does not cover (gather) run as transform + reduce
+  // on the device.  No counterpart in the reference, which always runs the two passes.
+  bool launch_transform_reduce(const std::string& map_kernel, const std::string& reduce_kernel,
+                               void* in, size_t count, size_t in_elem_size, size_t out_elem_size,
+                               void* out_result);
   size_t launch_argminmax(const std::string& kernel_name, void* data, size_t count,
                           size_t elem_size, bool is_float, bool want_max, bool want_last);
   size_t launch_find(const std::string& kernel_name, void* data, size_t count, size_t elem_size,

@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "parallax/descriptor.h"
+#include "parallax/runtime.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -82,6 +83,13 @@ int pxd_timer_end(pxd_ctx_t ctx, double* ms);
 /* ---- plain primitives (device pointers) */
 /* *d_out = sum of d_in[0..n) in the element type (one element written). */
 int pxd_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, void* d_out);
+/* Σ map(x) in one pass: `map` is a PX_OP_TRANSFORM descriptor (input dtype, output dtype
+ * in aux, map in fn: every map on the same type or widening i32->i64, u32->u64,
+ * f32->f64; converting identity between any two types; PX_MAP_PRED counts into an
+ * integer type) or a PX_OP_MAP (output type = input type).  One element of the output
+ * type to *d_out.  PX_EARG for maps the fused kernel does not cover (gather). */
+int pxd_transform_reduce(pxd_ctx_t ctx, const px_desc* map, const void* d_in, size_t n,
+                         void* d_out);
 int pxd_inclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n);
 /* h_init: host bytes of one element (NULL means 0). */
 int pxd_exclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n,
@@ -118,6 +126,16 @@ int pxb_seg_sort(pxd_ctx_t ctx, int dtype, void* d_data, const uint64_t* h_offse
  * receives the kept count of segment s.  flags->fn == PX_PRED_UNIQUE gives unique. */
 int pxb_seg_copy_if(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void* d_out,
                     const uint64_t* h_offsets, size_t nseg, uint64_t* d_counts);
+
+/* ---- funnel extension (no counterpart in the reference, which runs transform2 and
+ * reduce as separate calls through an arena array, stdpar.hpp:246-293): Σ map(x) in one
+ * device pass for the transform_reduce / count_if / all_of / any_of / none_of funnels.
+ * map_kernel: a TRANSFORM (or MAP) descriptor handle; reduce_kernel: a REDUCE of the
+ * map's output type.  Same conventions as parallax_reduce (void; failures reported on
+ * stderr; the result is host-visible on return). */
+void parallax_transform_reduce(parallax_kernel_t map_kernel, parallax_kernel_t reduce_kernel,
+                               void* input, size_t count, size_t in_elem_size,
+                               size_t out_elem_size, void* result);
 
 /* ---- graph plans: capture one batched call into a hipGraph once, replay it many
  * times (same buffers, same offsets).  op: PX_OP_REDUCE, PX_OP_SCAN, PX_OP_SORT,

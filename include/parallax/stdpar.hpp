@@ -15,6 +15,7 @@
 #ifndef PARALLAX_STDPAR_HPP
 #define PARALLAX_STDPAR_HPP
 
+#include <parallax/px_device.h>
 #include <parallax/runtime.h>
 
 #include <algorithm>
@@ -133,21 +134,17 @@ __attribute__((noinline)) void device_exclusive_scan(const T* in, T* out, std::s
   }
 }
 
-// ---- transform_reduce / count_if: a mapping kernel (":xform" / ":pred") into a
-// scratch array, then the sum (":reduce")
+// ---- transform_reduce / count_if: the mapping descriptor (":xform" / ":pred") and the
+// sum (":reduce") run as ONE fused device pass (parallax_transform_reduce); the reference
+// wrote f(x) to an arena array and reduced it (stdpar.hpp:246-293 there).
 template <class T, class U, class F>
 __attribute__((noinline)) U device_transform_reduce(const T* in, std::size_t n, F f) {
   static parallax_kernel_t kx = lookup_suffixed(__PRETTY_FUNCTION__, ":xform");
   static parallax_kernel_t kr = lookup_suffixed(__PRETTY_FUNCTION__, ":reduce");
   if (kx && kr && std::is_empty_v<F>) {
-    U* tmp = static_cast<U*>(parallax_arena_alloc(n * sizeof(U), alignof(U)));
-    if (tmp) {
-      parallax_kernel_launch_transform2(kx, const_cast<T*>(in), tmp, n, sizeof(T), sizeof(U));
-      U sum{};
-      parallax_reduce(kr, tmp, n, sizeof(U), &sum);
-      parallax_arena_free(tmp);
-      return sum;
-    }
+    U sum{};
+    parallax_transform_reduce(kx, kr, const_cast<T*>(in), n, sizeof(T), sizeof(U), &sum);
+    return sum;
   }
   U acc{};
   for (std::size_t i = 0; i < n; ++i) acc = acc + static_cast<U>(f(in[i]));
@@ -159,14 +156,9 @@ __attribute__((noinline)) long device_count_if(const T* in, std::size_t n, Pred 
   static parallax_kernel_t kp = lookup_suffixed(__PRETTY_FUNCTION__, ":pred");
   static parallax_kernel_t kr = lookup_suffixed(__PRETTY_FUNCTION__, ":reduce");
   if (kp && kr && std::is_empty_v<Pred>) {
-    int* tmp = static_cast<int*>(parallax_arena_alloc(n * sizeof(int), alignof(int)));
-    if (tmp) {
-      parallax_kernel_launch_transform2(kp, const_cast<T*>(in), tmp, n, sizeof(T), sizeof(int));
-      int c = 0;
-      parallax_reduce(kr, tmp, n, sizeof(int), &c);
-      parallax_arena_free(tmp);
-      return c;
-    }
+    int c = 0;  // the ":pred" / ":reduce" pair counts in int, as the reference registers it
+    parallax_transform_reduce(kp, kr, const_cast<T*>(in), n, sizeof(T), sizeof(int), &c);
+    return c;
   }
   long c = 0;
   for (std::size_t i = 0; i < n; ++i)

@@ -190,6 +190,15 @@ int pxd_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, void* d_out
   return parallax::run_reduce(*ctx->c, dtype, d_in, n, d_out);
 }
 
+int pxd_transform_reduce(pxd_ctx_t ctx, const px_desc* map, const void* d_in, size_t n,
+                         void* d_out) {
+  PXD_GUARD();
+  if (!map) return PX_EARG;
+  px_desc m = *map;
+  if (m.op == PX_OP_MAP) m.aux = m.dtype;
+  return parallax::run_map_reduce(*ctx->c, m, d_in, n, d_out);
+}
+
 int pxd_inclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n) {
   PXD_GUARD();
   return parallax::run_scan(*ctx->c, dtype, d_in, d_out, n, false, nullptr);

@@ -383,6 +383,17 @@ void parallax_reduce(parallax_kernel_t kernel, void* data, size_t count, size_t 
   if (l) l->launch_reduce(handle_of(kernel)->name, data, count, elem_size, result);
 }
 
+void parallax_transform_reduce(parallax_kernel_t map_kernel, parallax_kernel_t reduce_kernel,
+                               void* input, size_t count, size_t in_elem_size,
+                               size_t out_elem_size, void* result) {
+  std::lock_guard<std::recursive_mutex> lk(abi_mutex());
+  if (!map_kernel || !reduce_kernel) return fail("parallax_transform_reduce", "null kernel");
+  parallax::KernelLauncher* l = launcher_or_null("parallax_transform_reduce");
+  if (l)
+    l->launch_transform_reduce(handle_of(map_kernel)->name, handle_of(reduce_kernel)->name, input,
+                               count, in_elem_size, out_elem_size, result);
+}
+
 size_t parallax_argminmax(parallax_kernel_t kernel, void* data, size_t count, size_t elem_size,
                           int is_float, int want_max, int want_last) {
   std::lock_guard<std::recursive_mutex> lk(abi_mutex());

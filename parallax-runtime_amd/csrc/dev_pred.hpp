@@ -13,7 +13,7 @@
 namespace pxk {
 
 template <class T>
-__device__ __forceinline__ T bits_as(uint64_t b) {
+__host__ __device__ __forceinline__ T bits_as(uint64_t b) {
   if constexpr (sizeof(T) == 4) {
     return __builtin_bit_cast(T, static_cast<uint32_t>(b));
   } else {
@@ -63,7 +63,7 @@ struct Pred {
   uint32_t fn, flags;
   T c0, c1;
   T mod_mask;  // m - 1 when MOD_EQ's divisor m is a positive power of two, else 0
-  __device__ __forceinline__ Pred(const px_desc& d)
+  __host__ __device__ __forceinline__ Pred(const px_desc& d)
       : fn(d.fn), flags(d.flags), c0(bits_as<T>(d.c0)), c1(bits_as<T>(d.c1)), mod_mask(T(0)) {
     if constexpr (std::is_integral_v<T>) {
       if (fn == PX_PRED_MOD_EQ && c0 > T(0) && (c0 & (c0 - T(1))) == T(0)) mod_mask = c0 - T(1);

@@ -266,6 +266,19 @@ int run_reduce(StreamContext& c, int dtype, const void* in, uint64_t n, void* ou
   return dev_err(pxk::reduce(dtype, in, n, out_dev, c.scratch(), c.stream()), "reduce");
 }
 
+int run_map_reduce(StreamContext& c, const px_desc& map, const void* in, uint64_t n, void* out_dev) {
+  if (!valid_dtype(map.dtype) || !valid_dtype(map.aux) || !out_dev || (n && !in)) return PX_ERR_ARG;
+  if (!pxk::map_reduce_supported(map)) {
+    set_last_error(PX_ERR_ARG, "map_reduce: unsupported map for the fused reduce");
+    return PX_ERR_ARG;
+  }
+  ++g_stats.reduce;
+  ++g_stats.launches;
+  if (n == 0) return zero_dev(c, out_dev, pxk::dtype_size(map.aux));
+  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16)) return PX_ERR_NOMEM;
+  return dev_err(pxk::map_reduce(map, in, n, out_dev, c.scratch(), c.stream()), "map_reduce");
+}
+
 int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n, bool exclusive,
              const void* init_host) {
   if (!valid_dtype(dtype) || (n && (!in || !out))) return PX_ERR_ARG;

@@ -105,6 +105,8 @@ void build_tile_tables(const SegSpec& sp, uint32_t tile, uint32_t vec, int64_t s
 
 // ---- primitive runners on a context (device pointers, stream-ordered, no host sync)
 int run_reduce(StreamContext& c, int dtype, const void* in, uint64_t n, void* out_dev);
+// Fused Σ map(x) (pxk::map_reduce); the result (map.aux type) goes to out_dev.
+int run_map_reduce(StreamContext& c, const px_desc& map, const void* in, uint64_t n, void* out_dev);
 int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n, bool exclusive,
              const void* init_host);
 int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::LaunchHook& hook = {});

@@ -13,63 +13,13 @@
 #include <type_traits>
 
 #include "dev_common.hpp"
+#include "dev_map.hpp"
 #include "dev_pred.hpp"
 #include "kernels.hpp"
 
 namespace pxk {
 
 namespace {
-
-template <class U>
-__device__ __forceinline__ U wrap_add(U a, U b) {
-  if constexpr (std::is_integral_v<U>) {
-    using W = std::make_unsigned_t<U>;
-    return static_cast<U>(static_cast<W>(static_cast<W>(a) + static_cast<W>(b)));
-  } else {
-    return a + b;
-  }
-}
-template <class U>
-__device__ __forceinline__ U wrap_mul(U a, U b) {
-  if constexpr (std::is_integral_v<U>) {
-    using W = std::make_unsigned_t<U>;
-    return static_cast<U>(static_cast<W>(static_cast<W>(a) * static_cast<W>(b)));
-  } else {
-    return a * b;
-  }
-}
-
-template <class U>
-__device__ __forceinline__ U map_apply(uint32_t fn, U x, U a, U b) {
-  switch (fn) {
-    case PX_MAP_IDENTITY: return x;
-    case PX_MAP_FILL: return a;
-    case PX_MAP_ADD: return wrap_add(x, a);
-    case PX_MAP_MUL: return wrap_mul(x, a);
-    case PX_MAP_AXPB:
-      if constexpr (std::is_floating_point_v<U>) {
-        return __builtin_fma(a, x, b);  // single rounding
-      } else {
-        return wrap_add(wrap_mul(a, x), b);
-      }
-    case PX_MAP_NEG:
-      if constexpr (std::is_floating_point_v<U>) {
-        return -x;
-      } else {
-        return wrap_mul(x, static_cast<U>(-1));
-      }
-    case PX_MAP_ABS:
-      if constexpr (std::is_floating_point_v<U>) {
-        return __builtin_fabs(x);
-      } else if constexpr (std::is_signed_v<U>) {
-        return x < 0 ? wrap_mul(x, static_cast<U>(-1)) : x;
-      } else {
-        return x;
-      }
-    case PX_MAP_SQUARE: return wrap_mul(x, x);
-    default: return x;
-  }
-}
 
 template <class U>
 __device__ __forceinline__ U cap_or_const(const MapArgs& a, int slot, uint64_t c) {

@@ -1,19 +1,25 @@
-// k_reduce.hip -- sum reduction for gfx950.
+// k_reduce.hip -- sum reduction for gfx950, plain, fused with an element map, and
+// segmented.
 //
 // Plain form (reference: KernelLauncher::launch_reduce, src/kernel/kernel_launcher.cpp:908-985,
 // which runs reduce.comp level after level with a host wait between levels): here ONE
-// launch.  A grid sized to fill every CU streams the input with 16-byte loads, four
-// in flight per thread, accumulating in registers in the carry type; each wave folds
-// with DPP, waves fold through LDS, every block leaves one partial, and the last block
-// to finish (agent-scope counter) folds the partials in a fixed order and writes the
-// result.  The counter resets itself, so back-to-back calls need no reset launch.
+// launch.  A grid of one 256-thread block per CU streams the input with non-temporal
+// 16-byte loads, eight in flight per thread over two accumulator chains, in the carry
+// type; each wave folds with DPP, waves fold through LDS, every block leaves one
+// partial, and the last block to finish (agent-scope counter) folds the partials in a
+// fixed order and writes the result.  The counter resets itself, so back-to-back calls
+// need no reset launch.
 //
-// Segmented form: a persistent grid walks the tiles of all segments (static
-// assignment: no tile waits on another); single-tile segments write their sum
-// directly, larger ones leave per-tile partials and the last tile of the segment (a
-// self-resetting per-segment counter) folds them in tile order.  Segment s's sum goes
-// to out[seg_out[s]] (out[s] without an output table).
+// Fused map-reduce (transform_reduce / count_if): the same kernel with an element map
+// applied between the load and the add, so Σ f(x) costs one read of x (the reference
+// path, stdpar.hpp:246-293, writes f(x) to an arena array and reduces it: three passes).
+//
+// Segmented form: pass 1 walks the tiles of all segments (static assignment: no tile
+// waits on another); single-tile segments write their sum directly, larger ones leave
+// one partial per tile; pass 2 folds each segment's partials in tile order.  Segment s's
+// sum goes to out[seg_out[s]] (out[s] without an output table).
 #include "dev_common.hpp"
+#include "dev_map.hpp"
 #include "dev_tiles.hpp"
 #include "kernels.hpp"
 
@@ -35,13 +41,40 @@ __device__ __forceinline__ V load16(const V* p) {
   }
 }
 
-template <class T, int VEC, int UNROLL, bool NT>
-__global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int64_t lo,
+// Identity map (plain reduce).
+template <class T>
+struct MapNone {
+  __device__ __forceinline__ T operator()(T x) const { return x; }
+};
+
+// Adds map(v) for the VEC elements of one vector to acc.
+template <class TI, class TO, int VEC, class M>
+__device__ __forceinline__ void add_mapped(typename Traits<TO>::A& acc, const Vec<TI, VEC>& v,
+                                           const M& m) {
+  if constexpr (is_map_pred<M>::value) {
+    acc = acc_add(acc, to_acc(static_cast<TO>(__builtin_popcount(m.p.template mask<VEC>(v.v)))));
+  } else {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(m(v.v[j])));
+  }
+}
+
+template <class TI, class TO, class M>
+__device__ __forceinline__ TO map_one(TI x, const M& m) {
+  if constexpr (is_map_pred<M>::value) {
+    return m.p(x) ? TO(1) : TO(0);
+  } else {
+    return m(x);
+  }
+}
+
+template <class TI, class TO, int VEC, int UNROLL, bool NT, class M>
+__global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ in_al, int64_t lo,
                                                 int64_t hi,
-                                                typename Traits<T>::A* __restrict__ partials,
-                                                Ctl* ctl, T* out) {
-  using A = typename Traits<T>::A;
-  using V = Vec<T, VEC>;
+                                                typename Traits<TO>::A* __restrict__ partials,
+                                                Ctl* ctl, TO* out, M m) {
+  using A = typename Traits<TO>::A;
+  using V = Vec<TI, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
   __shared__ A s_w[4];
   __shared__ int s_last;
@@ -58,18 +91,13 @@ __global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int
     for (int u = 0; u < UNROLL; ++u) v[u] = load16<V, NT>(vin + i + u * stride);
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        if (u & 1) acc2 = acc_add(acc2, to_acc(v[u].v[j]));
-        else acc = acc_add(acc, to_acc(v[u].v[j]));
-      }
+      if (u & 1)
+        add_mapped<TI, TO, VEC>(acc2, v[u], m);
+      else
+        add_mapped<TI, TO, VEC>(acc, v[u], m);
     }
   }
-  for (; i < b; i += stride) {
-    const V v0 = load16<V, NT>(vin + i);
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v0.v[j]));
-  }
+  for (; i < b; i += stride) add_mapped<TI, TO, VEC>(acc, load16<V, NT>(vin + i), m);
   acc = acc_add(acc, acc2);
   if (blockIdx.x == 0 && threadIdx.x < 2 * VEC) {
     // head [lo, min(a*VEC, hi)) and tail [max(a,b)*VEC, hi): fewer than VEC each
@@ -82,7 +110,7 @@ __global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int
       pos = (a > b ? a : b) * VEC + (threadIdx.x - VEC);
       ok = pos < hi;
     }
-    if (ok) acc = acc_add(acc, to_acc(in_al[pos]));
+    if (ok) acc = acc_add(acc, to_acc(map_one<TI, TO>(in_al[pos], m)));
   }
 
   const int lane = lane_id();
@@ -108,7 +136,7 @@ __global__ __launch_bounds__(256) void k_reduce(const T* __restrict__ in_al, int
   __syncthreads();
   if (threadIdx.x == 0) {
     const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
-    *out = from_acc<T>(tot);
+    *out = from_acc<TO>(tot);
   }
 }
 
@@ -180,17 +208,18 @@ __global__ __launch_bounds__(256) void k_seg_combine(SegGeom g,
   if (lane_id() == 0) out[g.seg_out ? g.seg_out[seg] : seg] = from_acc<T>(w);
 }
 
-template <class T>
-hipError_t reduce_t(const void* in, uint64_t n, void* out_dev, const Scratch& s,
-                    hipStream_t stream) {
-  using A = typename Traits<T>::A;
-  constexpr int VEC = 16 / sizeof(T);
-  const T* in_al = reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
-  const int64_t lo = elem_shift(in, sizeof(T));
+// Launch of the (mapped) reduce over n elements of TI at `in` (any alignment).
+template <class TI, class TO, bool TUNABLE, class M>
+hipError_t reduce_launch(const void* in, uint64_t n, void* out_dev, const M& m, const Scratch& s,
+                         hipStream_t stream) {
+  using A = typename Traits<TO>::A;
+  constexpr int VEC = 16 / sizeof(TI);
+  const TI* in_al = reinterpret_cast<const TI*>(reinterpret_cast<uintptr_t>(in) & ~uintptr_t(15));
+  const int64_t lo = elem_shift(in, sizeof(TI));
   const int64_t hi = lo + static_cast<int64_t>(n);
   const int64_t nvec = hi / VEC - (lo + VEC - 1) / VEC;
   const Tune& t = tune();
-  const int unroll = t.red_unroll == 4 ? 4 : 8;
+  const int unroll = (TUNABLE && t.red_unroll == 4) ? 4 : 8;
   int64_t grid = (nvec + 256 * unroll - 1) / (256 * unroll);
   if (grid < 1) grid = 1;
   const int64_t cap = static_cast<int64_t>(s.max_blocks / 8) * (t.red_bpc > 0 ? t.red_bpc : 8);
@@ -198,16 +227,76 @@ hipError_t reduce_t(const void* in, uint64_t n, void* out_dev, const Scratch& s,
   if (static_cast<size_t>(grid) * sizeof(A) > s.partial_bytes) return hipErrorInvalidValue;
   const uint32_t gb = static_cast<uint32_t>(grid);
   A* part = reinterpret_cast<A*>(s.partials);
-  T* o = reinterpret_cast<T*>(out_dev);
-  if (unroll == 4 && t.red_nt)
-    k_reduce<T, VEC, 4, true><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
-  else if (unroll == 4)
-    k_reduce<T, VEC, 4, false><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
-  else if (t.red_nt)
-    k_reduce<T, VEC, 8, true><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
-  else
-    k_reduce<T, VEC, 8, false><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o);
+  TO* o = reinterpret_cast<TO*>(out_dev);
+  if (!TUNABLE || (unroll == 8 && t.red_nt)) {
+    k_reduce<TI, TO, VEC, 8, true><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o, m);
+    return hipGetLastError();
+  }
+  if constexpr (TUNABLE) {
+    if (unroll == 4 && t.red_nt)
+      k_reduce<TI, TO, VEC, 4, true><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o, m);
+    else if (unroll == 4)
+      k_reduce<TI, TO, VEC, 4, false><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o, m);
+    else
+      k_reduce<TI, TO, VEC, 8, false><<<gb, 256, 0, stream>>>(in_al, lo, hi, part, s.ctl, o, m);
+  }
   return hipGetLastError();
+}
+
+template <class T>
+hipError_t reduce_t(const void* in, uint64_t n, void* out_dev, const Scratch& s,
+                    hipStream_t stream) {
+  return reduce_launch<T, T, true>(in, n, out_dev, MapNone<T>{}, s, stream);
+}
+
+// Fused map-reduce dispatch: same-type maps, converting identity maps, predicate counts.
+template <class TI, class TO>
+hipError_t map_reduce_tt(const px_desc& d, const void* in, uint64_t n, void* out_dev,
+                         const Scratch& s, hipStream_t stream) {
+  if (d.fn == PX_MAP_PRED) {
+    if constexpr (std::is_integral_v<TO>) {
+      MapPred<TI, TO> m{Pred<TI>(px_desc{d.magic, d.version, d.op, d.dtype, d.aux, d.fn2, d.flags, 0,
+                                         d.c0, d.c1})};
+      return reduce_launch<TI, TO, false>(in, n, out_dev, m, s, stream);
+    }
+    return hipErrorInvalidValue;
+  }
+  const TO a = bits_as<TO>(d.c0), b = bits_as<TO>(d.c1);
+  if (d.fn == PX_MAP_IDENTITY)
+    return reduce_launch<TI, TO, false>(in, n, out_dev, MapOp<TI, TO, PX_MAP_IDENTITY>{a, b}, s, stream);
+  constexpr bool widening = (std::is_same_v<TI, int32_t> && std::is_same_v<TO, int64_t>) ||
+                            (std::is_same_v<TI, uint32_t> && std::is_same_v<TO, uint64_t>) ||
+                            (std::is_same_v<TI, float> && std::is_same_v<TO, double>);
+  if constexpr (std::is_same_v<TI, TO> || widening) {
+    switch (d.fn) {
+#define PXK_MR(F) \
+  case F: return reduce_launch<TI, TO, false>(in, n, out_dev, MapOp<TI, TO, F>{a, b}, s, stream);
+      PXK_MR(PX_MAP_FILL)
+      PXK_MR(PX_MAP_ADD)
+      PXK_MR(PX_MAP_MUL)
+      PXK_MR(PX_MAP_AXPB)
+      PXK_MR(PX_MAP_NEG)
+      PXK_MR(PX_MAP_ABS)
+      PXK_MR(PX_MAP_SQUARE)
+#undef PXK_MR
+      default: break;
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class TI>
+hipError_t map_reduce_t(const px_desc& d, const void* in, uint64_t n, void* out_dev,
+                        const Scratch& s, hipStream_t stream) {
+  switch (d.aux) {
+    case PX_I32: return map_reduce_tt<TI, int32_t>(d, in, n, out_dev, s, stream);
+    case PX_U32: return map_reduce_tt<TI, uint32_t>(d, in, n, out_dev, s, stream);
+    case PX_I64: return map_reduce_tt<TI, int64_t>(d, in, n, out_dev, s, stream);
+    case PX_U64: return map_reduce_tt<TI, uint64_t>(d, in, n, out_dev, s, stream);
+    case PX_F32: return map_reduce_tt<TI, float>(d, in, n, out_dev, s, stream);
+    case PX_F64: return map_reduce_tt<TI, double>(d, in, n, out_dev, s, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <class T>
@@ -240,6 +329,22 @@ hipError_t seg_reduce_t(const void* in, const SegGeom& g, void* out, const Scrat
 hipError_t reduce(int dtype, const void* in, uint64_t n, void* out_dev, const Scratch& s,
                   hipStream_t stream) {
   PXK_DISPATCH(dtype, return reduce_t<T>(in, n, out_dev, s, stream));
+  return hipErrorInvalidValue;
+}
+
+bool map_reduce_supported(const px_desc& d) {
+  if (d.dtype >= PX_NUM_DTYPES || d.aux >= PX_NUM_DTYPES) return false;
+  if (d.fn == PX_MAP_PRED) return d.aux <= PX_U64 && d.fn2 < PX_NUM_PREDS && d.fn2 != PX_PRED_UNIQUE;
+  if (d.fn == PX_MAP_IDENTITY) return true;
+  const bool widening = (d.dtype == PX_I32 && d.aux == PX_I64) || (d.dtype == PX_U32 && d.aux == PX_U64) ||
+                        (d.dtype == PX_F32 && d.aux == PX_F64);
+  return (d.dtype == d.aux || widening) && d.fn >= PX_MAP_FILL && d.fn <= PX_MAP_SQUARE;
+}
+
+hipError_t map_reduce(const px_desc& d, const void* in, uint64_t n, void* out_dev, const Scratch& s,
+                      hipStream_t stream) {
+  if (!map_reduce_supported(d)) return hipErrorInvalidValue;
+  PXK_DISPATCH(d.dtype, return map_reduce_t<T>(d, in, n, out_dev, s, stream));
   return hipErrorInvalidValue;
 }
 

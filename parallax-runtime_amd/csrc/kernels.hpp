@@ -108,6 +108,12 @@ hipError_t reduce(int dtype, const void* in, uint64_t n, void* out_dev, const Sc
                   hipStream_t stream);
 hipError_t seg_reduce(int dtype, const void* in, const SegGeom& g, void* out, const Scratch& s,
                       hipStream_t stream);
+// Fused Σ map(x): `map` is a PX_OP_TRANSFORM descriptor (input dtype, output dtype in
+// aux, map op in fn, predicate in fn2 for PX_MAP_PRED, constants c0 / c1).  Supported:
+// same-type maps, converting identity, predicate counts into an integer type.
+bool map_reduce_supported(const px_desc& map);
+hipError_t map_reduce(const px_desc& map, const void* in, uint64_t n, void* out_dev,
+                      const Scratch& s, hipStream_t stream);
 
 // ---- scan (k_scan.hip): inclusive, or exclusive with `init` (host bytes of one element)
 hipError_t scan(int dtype, const void* in, void* out, const SegGeom& g, bool exclusive,

@@ -330,6 +330,54 @@ bool KernelLauncher::launch_reduce(const std::string& kernel_name, void* data, s
   return ok;
 }
 
+bool KernelLauncher::launch_transform_reduce(const std::string& map_kernel,
+                                             const std::string& reduce_kernel, void* in,
+                                             size_t count, size_t in_elem_size,
+                                             size_t out_elem_size, void* out_result) {
+  const char* who = "launch_transform_reduce";
+  const px_desc* m = find(map_kernel, who);
+  const px_desc* r = find(reduce_kernel, who);
+  if (!m || !r) return false;
+  if ((m->op != PX_OP_TRANSFORM && m->op != PX_OP_MAP) || r->op != PX_OP_REDUCE)
+    return report(who, "need a transform and a reduce descriptor"), false;
+  px_desc md = *m;
+  if (md.op == PX_OP_MAP) md.aux = md.dtype;  // an in-place map keeps its type
+  if (r->dtype != md.aux) return report(who, "reduce type differs from the map's output type"), false;
+  if (!size_ok(md.dtype, in_elem_size) || !size_ok(md.aux, out_elem_size) || !out_result)
+    return report(who, "bad element size or result pointer"), false;
+  Operand op;
+  bool ok = resolve(in, count * in_elem_size, true, op);
+  if (ok) {
+    prof_begin();
+    if (pxk::map_reduce_supported(md)) {
+      ok = run_map_reduce(*ctx_, md, op.dev, count, dev_value_) == PX_OK;
+    } else {
+      // two passes on the device through a temporary array of the output type
+      void* tmp = nullptr;
+      ok = hipMalloc(&tmp, std::max<size_t>(count * out_elem_size, 16)) == hipSuccess;
+      if (ok) {
+        ok = run_transform(*ctx_, md, op.dev, tmp, count, nullptr, 0, arena_host_base(arena_),
+                           arena_dev_base(arena_), arena_bytes(arena_)) == PX_OK &&
+             run_reduce(*ctx_, md.aux, tmp, count, dev_value_) == PX_OK;
+        (void)hipStreamSynchronize(ctx_->stream());
+        (void)hipFree(tmp);
+      } else {
+        (void)hipGetLastError();
+        set_last_error(PX_ERR_NOMEM, "transform_reduce temporary");
+      }
+    }
+    prof_end("transform_reduce", md.dtype);
+  }
+  ok = finish(op, false) && ok;
+  release(op);
+  if (ok) {
+    const hipMemcpyKind kind = is_device_pointer(out_result) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    ok = hipMemcpy(out_result, dev_value_, out_elem_size, kind) == hipSuccess;
+  }
+  if (!ok) report(who, last_error_text());
+  return ok;
+}
+
 bool KernelLauncher::launch_scan(const std::string& scan_kernel, const std::string& add_kernel,
                                  void* data, size_t count, size_t elem_size) {
   const px_desc* s = find(scan_kernel, "launch_scan");

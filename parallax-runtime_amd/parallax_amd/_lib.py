@@ -40,6 +40,7 @@ _proto("parallax_kernel_load", _vp, _u32p, _sz)
 _proto("parallax_kernel_launch_transform2", None, _vp, _vp, _vp, _sz, _sz, _sz)
 _proto("parallax_kernel_launch_transform2_captures", None, _vp, _vp, _vp, _sz, _sz, _sz, _vp, _sz)
 _proto("parallax_reduce", None, _vp, _vp, _sz, _sz, _vp)
+_proto("parallax_transform_reduce", None, _vp, _vp, _vp, _sz, _sz, _sz, _vp)
 _proto("parallax_argminmax", _sz, _vp, _vp, _sz, _sz, _i, _i, _i)
 _proto("parallax_find", _sz, _vp, _vp, _sz, _sz, _i, _vp)
 _proto("parallax_mismatch", _sz, _vp, _vp, _vp, _sz, _sz)
@@ -79,6 +80,7 @@ _proto("pxd_sync", _i, _vp)
 _proto("pxd_timer_begin", _i, _vp)
 _proto("pxd_timer_end", _i, _vp, ctypes.POINTER(ctypes.c_double))
 _proto("pxd_reduce", _i, _vp, _i, _vp, _sz, _vp)
+_proto("pxd_transform_reduce", _i, _vp, _dp, _vp, _sz, _vp)
 _proto("pxd_inclusive_scan", _i, _vp, _i, _vp, _vp, _sz)
 _proto("pxd_exclusive_scan", _i, _vp, _i, _vp, _vp, _sz, _vp)
 _proto("pxd_sort", _i, _vp, _i, _vp, _sz)

@@ -19,6 +19,9 @@
 struct IsEven {
   bool operator()(int x) const { return x % 2 == 0; }
 };
+struct Sq {
+  long operator()(int x) const { return static_cast<long>(x) * x; }
+};
 
 static int failures = 0;
 static void check(bool ok, const char* what) {
@@ -40,6 +43,22 @@ int main() {
   parallax_kernel_register((k + ":scan").c_str(), reinterpret_cast<unsigned*>(&scan), PX_DESC_WORDS);
   parallax_kernel_register((k + ":add").c_str(), reinterpret_cast<unsigned*>(&add), PX_DESC_WORDS);
   parallax_kernel_register((k + ":scatter").c_str(), reinterpret_cast<unsigned*>(&scatter), PX_DESC_WORDS);
+
+  // count_if and transform_reduce with registered functors: one fused device pass each
+  static px_desc pred{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_TRANSFORM, PX_I32, PX_I32, PX_MAP_PRED, 0, PX_PRED_MOD_EQ, 2, 0};
+  static px_desc sum_i32{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_REDUCE, PX_I32, 0, 0, 0, 0, 0, 0};
+  static px_desc sq{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_TRANSFORM, PX_I32, PX_I64, PX_MAP_SQUARE, 0, 0, 0, 0};
+  static px_desc sum_i64{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_REDUCE, PX_I64, 0, 0, 0, 0, 0, 0};
+  const std::string kc(
+      "long int parallax::detail::device_count_if(const T*, std::size_t, Pred) "
+      "[with T = int; Pred = IsEven; std::size_t = long unsigned int]");
+  const std::string kt(
+      "U parallax::detail::device_transform_reduce(const T*, std::size_t, F) "
+      "[with T = int; U = long int; F = Sq; std::size_t = long unsigned int]");
+  parallax_kernel_register((kc + ":pred").c_str(), reinterpret_cast<unsigned*>(&pred), PX_DESC_WORDS);
+  parallax_kernel_register((kc + ":reduce").c_str(), reinterpret_cast<unsigned*>(&sum_i32), PX_DESC_WORDS);
+  parallax_kernel_register((kt + ":xform").c_str(), reinterpret_cast<unsigned*>(&sq), PX_DESC_WORDS);
+  parallax_kernel_register((kt + ":reduce").c_str(), reinterpret_cast<unsigned*>(&sum_i64), PX_DESC_WORDS);
 
   std::mt19937_64 rng(7);
   const size_t n = 1000003;
@@ -80,6 +99,13 @@ int main() {
   std::copy_if(vi.begin(), vi.end(), std::back_inserter(out_ref), IsEven{});
   check(static_cast<size_t>(oe - out.begin()) == out_ref.size() &&
             std::equal(out_ref.begin(), out_ref.end(), out.begin()), "copy_if IsEven");
+  // registered count_if / transform_reduce (fused map + sum on the device)
+  const long ce = parallax::count_if(parallax::par, vi.begin(), vi.end(), IsEven{});
+  check(ce == std::count_if(vi.begin(), vi.end(), IsEven{}), "count_if IsEven");
+  const long sq2 = parallax::transform_reduce(parallax::par, vi.begin(), vi.end(), 5L, std::plus<>{}, Sq{});
+  long sq_ref = 5;
+  for (int x : vi) sq_ref += static_cast<long>(x) * x;
+  check(sq2 == sq_ref, "transform_reduce Sq");
   // a capturing lambda has no descriptor: it stays on the CPU and is still correct
   const int t = 17;
   const long c = parallax::count_if(parallax::par, vi.begin(), vi.end(), [t](int x) { return x > t; });

@@ -32,4 +32,4 @@ def test_funnels_on_gpu():
     assert rc == 0, out + err
     assert "FAIL" not in out
     launches = int(out.strip().splitlines()[-1].split()[-1])
-    assert launches >= 7, out  # reduce x2, scans x2, sort, unique, copy_if ran on the device
+    assert launches >= 9, out  # reduce x2, scans x2, sort, unique, copy_if, count_if, transform_reduce
