@@ -217,8 +217,12 @@ def plain_suite(args):
                        "pct_hbm_peak": round(100 * nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1)}
     lib.pxr_prof_enable(0)
     import math
-    a32 = X["reduce_f32"].astype(np.float64)
-    exact = float(np.dot(a32, a32))
+    # reference: x*x is exact in f64; sum it in 80-bit extended precision, in chunks
+    exact_ld = np.longdouble(0)
+    for c0 in range(0, n28, 1 << 24):
+        y = X["reduce_f32"][c0:c0 + (1 << 24)].astype(np.float64)
+        exact_ld += np.sum(np.square(y).astype(np.longdouble))
+    exact = float(exact_ld)
     bnd = 2 * math.ceil(math.log2(n28)) * float(np.finfo(np.float64).eps) * exact
     assert abs(fres[0] - exact) <= bnd, ("square sum", fres[0], exact)
     even = int(np.count_nonzero((X["copy_if_i32"] & 1) == 0))

@@ -83,22 +83,31 @@ __global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ in_al, in
   const int64_t b = hi / VEC;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
   int64_t i = a + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  A acc = acc_zero<A>();
-  A acc2 = acc_zero<A>();  // two chains for more add-level parallelism
+  // four independent accumulator chains (a double-double add is a long dependent
+  // sequence), combined in a fixed order
+  constexpr int NCH = 4;
+  A ch[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) ch[c] = acc_zero<A>();
   for (; i + (UNROLL - 1) * stride < b; i += UNROLL * stride) {
     V v[UNROLL];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) v[u] = load16<V, NT>(vin + i + u * stride);
+    if constexpr (is_map_pred<M>::value) {
+      // one predicate evaluation (one branch on its kind) for the whole group
+      TI x[UNROLL * VEC];
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      if (u & 1)
-        add_mapped<TI, TO, VEC>(acc2, v[u], m);
-      else
-        add_mapped<TI, TO, VEC>(acc, v[u], m);
+      for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) x[u * VEC + j] = v[u].v[j];
+      ch[0] = acc_add(ch[0], to_acc(static_cast<TO>(__builtin_popcount(m.p.template mask<UNROLL * VEC>(x)))));
+    } else {
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) add_mapped<TI, TO, VEC>(ch[u % NCH], v[u], m);
     }
   }
-  for (; i < b; i += stride) add_mapped<TI, TO, VEC>(acc, load16<V, NT>(vin + i), m);
-  acc = acc_add(acc, acc2);
+  for (; i < b; i += stride) add_mapped<TI, TO, VEC>(ch[0], load16<V, NT>(vin + i), m);
+  A acc = acc_add(acc_add(ch[0], ch[1]), acc_add(ch[2], ch[3]));
   if (blockIdx.x == 0 && threadIdx.x < 2 * VEC) {
     // head [lo, min(a*VEC, hi)) and tail [max(a,b)*VEC, hi): fewer than VEC each
     int64_t pos;
@@ -222,7 +231,9 @@ hipError_t reduce_launch(const void* in, uint64_t n, void* out_dev, const M& m, 
   const int unroll = (TUNABLE && t.red_unroll == 4) ? 4 : 8;
   int64_t grid = (nvec + 256 * unroll - 1) / (256 * unroll);
   if (grid < 1) grid = 1;
-  const int64_t cap = static_cast<int64_t>(s.max_blocks / 8) * (t.red_bpc > 0 ? t.red_bpc : 8);
+  // predicate counts do more ALU work per byte: two blocks per CU hide it (measured)
+  const int bpc = is_map_pred<M>::value ? 2 : (t.red_bpc > 0 ? t.red_bpc : 8);
+  const int64_t cap = static_cast<int64_t>(s.max_blocks / 8) * bpc;
   if (grid > cap) grid = cap;
   if (static_cast<size_t>(grid) * sizeof(A) > s.partial_bytes) return hipErrorInvalidValue;
   const uint32_t gb = static_cast<uint32_t>(grid);
