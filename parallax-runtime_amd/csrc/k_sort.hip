@@ -235,7 +235,11 @@ template <class T>
 struct SortCfg {
   static constexpr int kTileLog = (sizeof(T) == 4) ? 14 : 13;  // 64 KiB of LDS
   static constexpr int kThreads = 1 << (kTileLog - kGroupLog);  // one group per thread
-  static constexpr int kMaxC = 5;
+  static constexpr int kMaxC = (sizeof(T) == 4) ? 7 : 6;  // register pass: 2^C elements per thread
+  static int max_c() {
+    const int c = sizeof(T) == 4 ? tune().sort_c4 : tune().sort_c8;
+    return c < 1 ? 1 : (c > kMaxC ? kMaxC : c);
+  }
 };
 
 // MERGE == false: full sort of each tile (levels 1 .. tl).  MERGE == true: the
@@ -316,20 +320,18 @@ __global__ __launch_bounds__(256) void k_sort_global(T* data, uint64_t n, const 
   T* d = data + sg.base;
   const uint64_t lim = sg.n;
 
+  // Element q: group bits b-C+1 .. b-1 are q's low C-1 bits (contiguous), the top one
+  // selects irep or irep ^ m_top (the flip mask on a level's first pass).  Indices are
+  // recomputed at the store instead of held in registers (C = 7 holds 128 elements).
+  const uint64_t m_top = FLIP ? ((1ull << kl) - 1) : (1ull << b);
+  auto elem = [&](int q) -> uint64_t {
+    return irep ^ (static_cast<uint64_t>(q & (Q / 2 - 1)) << lowb) ^ ((q & (Q / 2)) ? m_top : 0ull);
+  };
   T v[Q];
-  uint64_t e[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    uint64_t x = 0;
-#pragma unroll
-    for (int r = 0; r < C; ++r) {
-      if ((q >> (C - 1 - r)) & 1) {
-        const uint64_t m = (FLIP && r == 0) ? ((1ull << kl) - 1) : (1ull << (b - r));
-        x ^= m;
-      }
-    }
-    e[q] = irep ^ x;
-    v[q] = e[q] < lim ? d[e[q]] : pos_inf<T>();
+    const uint64_t e = elem(q);
+    v[q] = e < lim ? d[e] : pos_inf<T>();
   }
 #pragma unroll
   for (int r = 0; r < C; ++r) {
@@ -346,9 +348,17 @@ __global__ __launch_bounds__(256) void k_sort_global(T* data, uint64_t n, const 
         cmpx(v[q2], v[q]);
     }
   }
+  // A copy of irep the compiler cannot see through (an empty asm that only claims the
+  // register): the store indices are recomputed from it instead of kept alive from the
+  // loads, which would cost two registers per element.
+  uint64_t irep_s = irep;
+  asm volatile("" : "+v"(irep_s));
 #pragma unroll
-  for (int q = 0; q < Q; ++q)
-    if (e[q] < lim) d[e[q]] = v[q];
+  for (int q = 0; q < Q; ++q) {
+    const uint64_t e =
+        irep_s ^ (static_cast<uint64_t>(q & (Q / 2 - 1)) << lowb) ^ ((q & (Q / 2)) ? m_top : 0ull);
+    if (e < lim) d[e] = v[q];
+  }
 }
 
 
@@ -372,17 +382,23 @@ hipError_t launch_global(int c, bool flip, uint32_t blocks, T* data, uint64_t n,
                          const uint32_t* blk_first, uint32_t b, uint32_t kl, hipStream_t stream) {
 #define PXK_G(CC)                                                                                \
   case CC:                                                                                       \
-    if (flip)                                                                                    \
-      k_sort_global<T, CC, true><<<blocks, 256, 0, stream>>>(data, n, seg_off, blk_seg, blk_first, b, kl); \
-    else                                                                                         \
-      k_sort_global<T, CC, false><<<blocks, 256, 0, stream>>>(data, n, seg_off, blk_seg, blk_first, b, kl); \
-    break;
+    if constexpr (CC <= SortCfg<T>::kMaxC) {                                                     \
+      if (flip)                                                                                  \
+        k_sort_global<T, CC, true><<<blocks, 256, 0, stream>>>(data, n, seg_off, blk_seg, blk_first, b, kl); \
+      else                                                                                       \
+        k_sort_global<T, CC, false><<<blocks, 256, 0, stream>>>(data, n, seg_off, blk_seg, blk_first, b, kl); \
+      break;                                                                                     \
+    } else {                                                                                     \
+      return hipErrorInvalidValue;                                                               \
+    }
   switch (c) {
     PXK_G(1)
     PXK_G(2)
     PXK_G(3)
     PXK_G(4)
     PXK_G(5)
+    PXK_G(6)
+    PXK_G(7)
     default: return hipErrorInvalidValue;
   }
 #undef PXK_G
@@ -414,7 +430,7 @@ hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream, const LaunchHook
     uint32_t b = kl - 1;
     bool first = true;
     while (remaining > 0) {
-      const int c = std::min(remaining, SortCfg<T>::kMaxC);
+      const int c = std::min(remaining, SortCfg<T>::max_c());
       const uint64_t groups = (1ull << nplog) >> c;
       const uint64_t blocks = (groups + 255) / 256;
       hook("sort_global", 0);
@@ -442,7 +458,7 @@ hipError_t sort_t(void* data_v, uint64_t n, hipStream_t stream, const LaunchHook
 template <class T>
 bool build_plan(const uint64_t* lo, const uint64_t* hi, uint32_t nseg, SegSortPlan& p) {
   constexpr int TL = SortCfg<T>::kTileLog;
-  constexpr int MC = SortCfg<T>::kMaxC;
+  const int MC = SortCfg<T>::max_c();
   p.nseg = nseg;
   p.tile_log = TL;
   p.max_c = MC;
@@ -493,7 +509,7 @@ hipError_t seg_sort_run_t(const SegSortPlan& p, void* data_v, const uint64_t* of
     uint32_t b = kl - 1;
     bool first = true;
     while (remaining > 0) {
-      const int c = std::min(remaining, SortCfg<T>::kMaxC);
+      const int c = std::min(remaining, p.max_c);
       if (p.gblocks[c]) {
         e = launch_global<T>(c, first, p.gblocks[c], data, 0, off_d, w + p.o_gseg[c],
                              w + p.o_gfirst[c], b, kl, stream);

@@ -50,6 +50,8 @@ struct Tune {
   int red_unroll = 8;         // 16-byte loads in flight per thread in the plain reduce
   int red_bpc = 1;            // blocks per CU of the plain reduce
   int red_nt = 1;             // non-temporal loads in the plain reduce
+  int sort_c4 = 6;            // stages per HBM register pass of the sort, 4-byte keys (<= 7)
+  int sort_c8 = 6;            // ... 8-byte keys (<= 6)
 };
 Tune& tune();
 bool tile_cfg_supported(TileCfg c);

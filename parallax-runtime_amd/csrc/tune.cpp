@@ -79,6 +79,8 @@ Tune load() {
     else if (k == "red_unroll") t.red_unroll = v;
     else if (k == "red_bpc") t.red_bpc = v;
     else if (k == "red_nt") t.red_nt = v;
+    else if (k == "sort_c4" && v >= 1 && v <= 7) t.sort_c4 = v;
+    else if (k == "sort_c8" && v >= 1 && v <= 6) t.sort_c8 = v;
   }
   for (int d = 0; d < PX_NUM_DTYPES; ++d) {
     TileCfg a{scan.block ? scan.block : t.scan[d].block, scan.k ? scan.k : t.scan[d].k};
