@@ -22,6 +22,28 @@ namespace pxk {
 
 namespace {
 
+// Writes the staged run src[0, count) to dst (any element alignment): a scalar head up
+// to the next 16-byte boundary, 16-byte non-temporal stores for the body, a scalar tail.
+template <class T, int BLOCK, class C>
+__device__ __forceinline__ void write_run(T* dst, const T* src, C count) {
+  constexpr int VEC = 16 / sizeof(T);
+  using V = Vec<T, VEC>;
+  const uint32_t mis = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(dst) & 15u) / sizeof(T));
+  uint32_t head = (VEC - mis) % VEC;
+  if (head > count) head = static_cast<uint32_t>(count);
+  if (threadIdx.x < head) dst[threadIdx.x] = src[threadIdx.x];
+  const uint32_t nvec = static_cast<uint32_t>((count - head) / VEC);
+  V* vd = reinterpret_cast<V*>(dst + head);
+  for (uint32_t v = threadIdx.x; v < nvec; v += BLOCK) {
+    V y;
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) y.v[j] = src[head + v * VEC + j];
+    st_stream(vd + v, y);
+  }
+  const uint32_t done = head + nvec * VEC;
+  if (threadIdx.x < count - done) dst[done + threadIdx.x] = src[done + threadIdx.x];
+}
+
 template <class T, int BLOCK, int K, class C, int MODE, bool UNIQUE>
 __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_shift, T* out,
                                                  SegGeom g, px_desc pd, uint64_t* counts,
@@ -182,14 +204,12 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     // ---- coalesced writes of the staged run(s)
     const uint64_t P = s_prefix;
     const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
-    T* o = out + seg_out + P;
-    for (uint32_t j = threadIdx.x; j < tile_kept; j += BLOCK) o[j] = s_stage[j];
+    write_run<T, BLOCK>(out + seg_out + P, s_stage, tile_kept);
     if (MODE == PX_SCATTER_PARTITION) {
       const uint64_t num_true = *num_true_dev;
       const uint64_t rej_before = static_cast<uint64_t>(ti.w0 + vstart_r - ti.lo) - P;
-      T* orj = out + seg_out + num_true + rej_before;
-      for (uint32_t j = tile_kept + threadIdx.x; j < tile_valid; j += BLOCK)
-        orj[j - tile_kept] = s_stage[j];
+      write_run<T, BLOCK>(out + seg_out + num_true + rej_before, s_stage + tile_kept,
+                          tile_valid - tile_kept);
     }
     __syncthreads();
   }
