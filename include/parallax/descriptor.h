@@ -52,7 +52,31 @@ enum px_op {
   PX_OP_TRANSFORM = 9,  /* out[i] = f(in[i]) with in dtype w[3], out dtype w[4] */
   PX_OP_ARGMINMAX = 10, /* search family */
   PX_OP_FIND = 11,
-  PX_OP_MISMATCH = 12
+  PX_OP_MISMATCH = 12,
+  PX_OP_CODE = 13       /* a gfx950 code object carried in the payload (below) */
+};
+
+/* Code-object payload (op PX_OP_CODE), for functors no built-in map describes (a
+ * compiler registers the gfx950 code object it emitted for the funnel's lambda):
+ *   words 0..11  the descriptor header: dtype = input element type, aux = output element
+ *                type, fn = px_code_kind, fn2 = length in bytes of the kernel symbol
+ *                name, c0 = size in bytes of the code object, c1 = 0;
+ *   then         the symbol name (no terminating NUL), zero-padded to a word boundary;
+ *   then         the code object (an HSACO for gfx950), zero-padded to a word boundary.
+ * The kernel is launched with 256-thread blocks and ceil(count / 256) blocks, one element
+ * per thread, on the runtime's stream, with these extern "C" signatures:
+ *   PX_CODE_MAP        (launch, launch_with_captures: in place)
+ *     void k(void* data, uint64_t count, const void* captures, uint64_t capture_size,
+ *            uint64_t host_base, uint64_t dev_base);
+ *   PX_CODE_TRANSFORM  (launch_transform2 / _captures: in -> out)
+ *     void k(const void* in, void* out, uint64_t count, const void* captures,
+ *            uint64_t capture_size, uint64_t host_base, uint64_t dev_base);
+ * `captures` is a device copy of the closure bytes (null when there are none);
+ * host_base / dev_base relocate a captured arena pointer p: dev_base + (p - host_base)
+ * (both 0 without an arena). */
+enum px_code_kind {
+  PX_CODE_MAP = 0,
+  PX_CODE_TRANSFORM = 1
 };
 
 /* Compaction scatter modes (w[4] of a PX_OP_SCATTER descriptor). */

@@ -103,6 +103,16 @@ class KernelLauncher {
   uint64_t* dev_word_ = nullptr;  // device scratch for counts / indices
   void* dev_value_ = nullptr;     // device scratch for a reduce result
   std::unordered_map<std::string, px_desc> kernels_;
+  struct CodeKernel {
+    hipModule_t module = nullptr;
+    hipFunction_t function = nullptr;
+  };
+  std::unordered_map<std::string, CodeKernel> code_;  // PX_OP_CODE payloads, by kernel name
+  void* dev_caps_ = nullptr;  // device copy of the captures of a code-object launch
+  size_t dev_caps_cap_ = 0;
+  bool load_code(const std::string& name, const px_desc& d, const uint32_t* words, size_t nwords);
+  bool run_code(const std::string& name, const void* in, void* out, uint64_t count,
+                const void* captures, size_t capture_size, bool transform);
   bool prof_on_ = false;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   std::unordered_map<std::string, std::pair<double, uint64_t>> prof_;

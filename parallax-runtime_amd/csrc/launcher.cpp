@@ -69,7 +69,18 @@ KernelLauncher::~KernelLauncher() {
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   if (dev_word_) (void)hipFree(dev_word_);
   if (dev_value_) (void)hipFree(dev_value_);
+  if (dev_caps_) (void)hipFree(dev_caps_);
+  for (auto& kv : code_)
+    if (kv.second.module) (void)hipModuleUnload(kv.second.module);
 }
+
+namespace {
+uint64_t arena_host_base(UnifiedArena* a) {
+  return (a && a->host_mirrored()) ? reinterpret_cast<uint64_t>(a->host_base()) : 0;
+}
+uint64_t arena_dev_base(UnifiedArena* a) { return a ? a->device_address() : 0; }
+uint64_t arena_bytes(UnifiedArena* a) { return (a && a->host_mirrored()) ? a->capacity() : 0; }
+}  // namespace
 
 bool KernelLauncher::load_kernel(const std::string& name, const uint32_t* words, size_t nwords) {
   if (!words || nwords < PX_DESC_WORDS) return false;
@@ -77,6 +88,7 @@ bool KernelLauncher::load_kernel(const std::string& name, const uint32_t* words,
   std::memcpy(&d, words, sizeof(d));
   if (d.magic != PX_DESC_MAGIC || d.version != PX_DESC_VERSION) return false;
   if (d.dtype >= PX_NUM_DTYPES) return false;
+  if (d.op == PX_OP_CODE) return load_code(name, d, words, nwords);
   if (d.op == PX_OP_NONE || d.op > PX_OP_MISMATCH) return false;
   if (d.op == PX_OP_TRANSFORM && d.aux >= PX_NUM_DTYPES) return false;
   if ((d.op == PX_OP_MAP || d.op == PX_OP_TRANSFORM) && d.fn >= PX_NUM_MAPOPS) return false;
@@ -84,6 +96,94 @@ bool KernelLauncher::load_kernel(const std::string& name, const uint32_t* words,
   if (d.op == PX_OP_SCATTER && d.aux > PX_SCATTER_PARTITION) return false;
   kernels_[name] = d;
   if (debug_on()) std::fprintf(stderr, "[parallax] load %s op=%u dtype=%u\n", name.c_str(), d.op, d.dtype);
+  return true;
+}
+
+// A code-object payload: header, symbol name, HSACO (descriptor.h).  The module is
+// loaded once and kept for the life of the process, like the handles.
+bool KernelLauncher::load_code(const std::string& name, const px_desc& d, const uint32_t* words,
+                               size_t nwords) {
+  if (d.aux >= PX_NUM_DTYPES || d.fn > PX_CODE_TRANSFORM) return false;
+  const size_t name_len = d.fn2;
+  const uint64_t code_size = d.c0;
+  if (name_len == 0 || name_len > 1024 || code_size == 0) return false;
+  const size_t name_words = (name_len + 3) / 4;
+  const size_t code_words = static_cast<size_t>((code_size + 3) / 4);
+  if (nwords < PX_DESC_WORDS + name_words + code_words) return false;
+  const char* nm = reinterpret_cast<const char*>(words + PX_DESC_WORDS);
+  const std::string symbol(nm, name_len);
+  // copy the image to word-aligned storage of its own (the payload may be unaligned
+  // for the loader's purposes only in theory; a copy also outlives the caller's words)
+  std::vector<uint32_t> image(words + PX_DESC_WORDS + name_words,
+                              words + PX_DESC_WORDS + name_words + code_words);
+  CodeKernel k;
+  if (hipModuleLoadData(&k.module, image.data()) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (hipModuleGetFunction(&k.function, k.module, symbol.c_str()) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipModuleUnload(k.module);
+    return false;
+  }
+  code_[name] = k;
+  kernels_[name] = d;
+  if (debug_on())
+    std::fprintf(stderr, "[parallax] load %s code object %s (%llu bytes)\n", name.c_str(),
+                 symbol.c_str(), static_cast<unsigned long long>(code_size));
+  return true;
+}
+
+bool KernelLauncher::run_code(const std::string& name, const void* in, void* out, uint64_t count,
+                              const void* captures, size_t capture_size, bool transform) {
+  auto it = code_.find(name);
+  if (it == code_.end()) return false;
+  if (count == 0) return true;
+  void* caps = nullptr;
+  if (captures && capture_size) {
+    if (capture_size > dev_caps_cap_) {
+      if (dev_caps_) (void)hipFree(dev_caps_);
+      dev_caps_ = nullptr;
+      dev_caps_cap_ = 0;
+      if (hipMalloc(&dev_caps_, capture_size) != hipSuccess) {
+        (void)hipGetLastError();
+        dev_caps_ = nullptr;
+        return false;
+      }
+      dev_caps_cap_ = capture_size;
+    }
+    if (hipMemcpyAsync(dev_caps_, captures, capture_size, hipMemcpyHostToDevice, ctx_->stream()) !=
+        hipSuccess)
+      return false;
+    caps = dev_caps_;
+  }
+  // Captured pointers may reach any part of the arena: with captures, the whole used
+  // arena goes to the device before the launch and comes back after it (the reference
+  // synchronises the used arena around every op).
+  const bool sync_arena = caps && arena_ && arena_->host_mirrored();
+  if (sync_arena) arena_->flush_to_device();
+  uint64_t n = count, cs = caps ? capture_size : 0;
+  uint64_t hb = arena_host_base(arena_), db = arena_dev_base(arena_);
+  const void* a_in = in;
+  void* a_out = out;
+  void* a_data = out;
+  void* args_map[] = {&a_data, &n, &caps, &cs, &hb, &db};
+  void* args_tr[] = {&a_in, &a_out, &n, &caps, &cs, &hb, &db};
+  const uint64_t blocks = (count + 255) / 256;
+  if (blocks > 0x7fffffffull) return false;
+  const hipError_t e =
+      hipModuleLaunchKernel(it->second.function, static_cast<unsigned>(blocks), 1, 1, 256, 1, 1, 0,
+                            ctx_->stream(), transform ? args_tr : args_map, nullptr);
+  if (e != hipSuccess) {
+    set_last_error(PX_ERR_DEVICE, std::string("code-object launch: ") + hipGetErrorString(e));
+    return false;
+  }
+  if (sync_arena) {
+    if (hipStreamSynchronize(ctx_->stream()) != hipSuccess) return false;
+    arena_->invalidate_from_device();
+  }
+  ++stats().transform;
+  ++stats().launches;
   return true;
 }
 
@@ -215,13 +315,6 @@ void KernelLauncher::prof_end(const char* op, uint32_t dtype) {
   ev_used_ = 0;
 }
 
-namespace {
-uint64_t arena_host_base(UnifiedArena* a) {
-  return (a && a->host_mirrored()) ? reinterpret_cast<uint64_t>(a->host_base()) : 0;
-}
-uint64_t arena_dev_base(UnifiedArena* a) { return a ? a->device_address() : 0; }
-uint64_t arena_bytes(UnifiedArena* a) { return (a && a->host_mirrored()) ? a->capacity() : 0; }
-}  // namespace
 
 bool KernelLauncher::launch(const std::string& kernel_name, void* buffer, size_t count,
                             float multiplier, size_t elem_size) {
@@ -249,6 +342,17 @@ bool KernelLauncher::launch_with_captures(const std::string& kernel_name, void* 
                                           size_t elem_size) {
   const px_desc* d = find(kernel_name, "launch");
   if (!d) return false;
+  if (d->op == PX_OP_CODE) {
+    if (d->fn != PX_CODE_MAP) return report("launch", "code object is not an in-place map"), false;
+    if (!size_ok(d->dtype, elem_size)) return report("launch", "elem_size does not match the descriptor"), false;
+    Operand op;
+    bool ok = resolve(buffer, count * elem_size, true, op);
+    if (ok) ok = run_code(kernel_name, op.dev, op.dev, count, captures, capture_size, false);
+    ok = finish(op, ok) && ok;
+    release(op);
+    if (!ok) report("launch", last_error_text());
+    return ok;
+  }
   if (d->op != PX_OP_MAP) return report("launch", "descriptor is not an in-place map"), false;
   if (!size_ok(d->dtype, elem_size))
     return report("launch", "elem_size does not match the descriptor"), false;
@@ -279,6 +383,22 @@ bool KernelLauncher::launch_transform(const std::string& kernel_name, void* in_b
                                       size_t out_elem_size, void* captures, size_t capture_size) {
   const px_desc* d = find(kernel_name, "launch_transform");
   if (!d) return false;
+  if (d->op == PX_OP_CODE) {
+    if (d->fn != PX_CODE_TRANSFORM)
+      return report("launch_transform", "code object is not a transform"), false;
+    if (out_elem_size == 0) out_elem_size = elem_size;
+    if (!size_ok(d->dtype, elem_size) || !size_ok(d->aux, out_elem_size))
+      return report("launch_transform", "element sizes do not match the descriptor"), false;
+    Operand in, out;
+    bool ok = resolve(in_buffer, count * elem_size, true, in) &&
+              resolve(out_buffer, count * out_elem_size, false, out);
+    if (ok) ok = run_code(kernel_name, in.dev, out.dev, count, captures, capture_size, true);
+    ok = finish(out, ok) && ok;
+    release(in);
+    release(out);
+    if (!ok) report("launch_transform", last_error_text());
+    return ok;
+  }
   px_desc m = *d;
   if (m.op == PX_OP_MAP) m.aux = m.dtype;
   if (m.op != PX_OP_TRANSFORM && m.op != PX_OP_MAP)
