@@ -9,10 +9,11 @@
 // Tile = BLOCK threads x K chunks x one 16-byte vector per lane.  Wave w owns K
 // consecutive 64-vector chunks; per chunk each lane sums its vector, the wave scans the
 // 64 sums with DPP and carries the running total to the next chunk; the wave totals meet
-// in LDS.  Tiles take tickets in order; each publishes its aggregate, wave 0
-// looks back over up to 64 predecessors per step (decoupled look-back), publishes its
-// inclusive prefix, and the tile writes its outputs with 16-byte stores.  Tiles of a
-// batched (segmented) call look back only inside their own segment.
+// in LDS.  Tiles take tickets in order; each publishes its aggregate, wave 0 looks back
+// (decoupled look-back: one predecessor status word per lane and poll, stopping at the
+// nearest inclusive prefix), publishes its inclusive prefix, and the tile writes its
+// outputs with non-temporal 16-byte stores.  Tiles of a batched (segmented) call look
+// back only inside their own segment.
 #include <cstring>
 
 #include "dev_common.hpp"

@@ -16,7 +16,8 @@
 //     with the tile resident in LDS (the tile sort, then one merge pass per level);
 //   * the remaining (global) stages of a level run C at a time in registers: a thread
 //     loads the 2^C elements that the C stages exchange among themselves, applies the
-//     C stages, and stores them back -- one HBM pass for up to 5 stages.
+//     C stages, and stores them back -- one HBM pass for up to 6 stages (7 compiled for
+//     4-byte keys; the default is set by the launch-shape configuration).
 //
 // Segmented form: the same kernels walk a block table, so thousands of segments of
 // different lengths sort in the same few launches (each segment virtually padded to its
