@@ -151,7 +151,7 @@ pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flag
                            uint64_t* d_counts);
 /* Force the tile shape (threads per block, 16-byte vectors per thread) of one op and
  * dtype for the calls that follow: PX_OP_SCAN and PX_OP_FLAGS (plain and batched; shapes
- * 256x8, 512x8, 1024x4, 1024x8) and PX_OP_REDUCE (batched; 256x4, 256x8, 256x16).
+ * 256x4, 256x8, 512x8, 1024x4, 1024x8) and PX_OP_REDUCE (batched; 256x4, 256x8, 256x16).
  * block = items = 0 restores the automatic choice.  Not synchronised with calls in
  * flight on other threads.  Plans created while a shape is forced skip the autotuner. */
 int pxr_tile_override(int op, int dtype, int block, int items);

@@ -379,7 +379,7 @@ std::vector<pxk::TileCfg> seg_tile_candidates(int op, int dtype) {
   if (op == PX_OP_REDUCE) {
     for (int k : {4, 8, 16}) v.push_back({256, k});
   } else {
-    for (pxk::TileCfg t : {pxk::TileCfg{256, 8}, pxk::TileCfg{512, 8}, pxk::TileCfg{1024, 4},
+    for (pxk::TileCfg t : {pxk::TileCfg{256, 4}, pxk::TileCfg{256, 8}, pxk::TileCfg{512, 8}, pxk::TileCfg{1024, 4},
                            pxk::TileCfg{1024, 8}})
       if (pxk::tile_cfg_supported(t) && !(dtype == PX_F64 && t.block == 1024 && t.k == 8)) v.push_back(t);
   }

@@ -264,6 +264,7 @@ hipError_t launch_compact_t(hipStream_t stream, const T* in_al, int64_t in_shift
         in_al, in_shift, out, g, pd, counts, num_true_dev, s.status, s.ctl);                  \
     return hipGetLastError();                                                                 \
   }
+  PXK_CMP(256, 4)
   PXK_CMP(256, 8)
   PXK_CMP(512, 8)
   PXK_CMP(1024, 4)

@@ -192,6 +192,7 @@ hipError_t scan_t(const void* in, void* out, const SegGeom& g, bool exclusive, c
       k_scan<T, B, KK, false><<<grid, B, 0, stream>>>(in_al, in_shift, o, out_shift, g, iv, s.status, s.ctl); \
     return hipGetLastError();                                                                      \
   }
+  PXK_SCAN(256, 4)
   PXK_SCAN(256, 8)
   PXK_SCAN(512, 8)
   PXK_SCAN(1024, 4)

@@ -8,7 +8,7 @@
 namespace pxk {
 
 bool tile_cfg_supported(TileCfg c) {
-  return (c.block == 256 && c.k == 8) || (c.block == 512 && c.k == 8) ||
+  return (c.block == 256 && (c.k == 4 || c.k == 8)) || (c.block == 512 && c.k == 8) ||
          (c.block == 1024 && (c.k == 4 || c.k == 8));
 }
 

@@ -147,7 +147,7 @@ def test_alone_equals_in_the_mix(gpu_ctx, dt):
             assert alone.tobytes() == ref.tobytes()
 
 
-SCAN_TILES = [(256, 8), (512, 8), (1024, 4), (1024, 8)]
+SCAN_TILES = [(256, 4), (256, 8), (512, 8), (1024, 4), (1024, 8)]
 REDUCE_TILES = [(256, 4), (256, 8), (256, 16)]
 
 
