@@ -150,17 +150,20 @@ def plain_suite(args):
                      "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 1)}
         step_bytes += bytes_
     # Kernels, one launch at a time (live HIP events on the launch stream): the
-    # single-launch primitives, and the three kernels a sort is made of, each of which
-    # reads and writes the whole array once per launch (2*n*s algorithmic bytes).
+    # single-launch primitives, and the kernels a sort is made of: the radix digit
+    # histograms (one read, n*s algorithmic bytes) and digit passes (one read + write,
+    # 2*n*s), or for the bitonic path its tile sort, register passes and LDS merges
+    # (2*n*s each).
     kern = {}
     for key, name, bytes_ in prims:
         if not key.startswith("sort"):
             tot, calls = prof(key)
             kern[name] = (tot, calls, bytes_)
     for dt, n, s in (("u32", n26, 4), ("f64", n25, 8)):
-        for k in ("sort_tile", "sort_global", "sort_merge"):
+        for k, nb in (("sort_hist", n * s), ("sort_radix", 2 * n * s), ("sort_tile", 2 * n * s),
+                      ("sort_global", 2 * n * s), ("sort_merge", 2 * n * s)):
             tot, calls = prof(f"{k}:{dt}")
-            kern[f"{k}_{dt}"] = (tot, calls, 2 * n * s)
+            kern[f"{k}_{dt}"] = (tot, calls, nb)
     lib.pxr_prof_enable(0)
     kernels = {}
     for name, (tot, calls, bytes_) in kern.items():

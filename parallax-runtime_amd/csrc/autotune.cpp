@@ -59,7 +59,7 @@ void load_file(Cache& c) {
   while (std::fscanf(fp, "%d %d %d %d %d %d %lf", &op, &dt, &a, &b, &blk, &k, &ms) == 7) {
     const pxk::TileCfg t{blk, k};
     const bool ok = op == PX_OP_REDUCE ? (blk == 256 && (k == 4 || k == 8 || k == 16))
-                                       : pxk::tile_cfg_supported(t);
+                                       : pxk::tile_cfg_supported(t, op);
     if (ok) c.map[Key{op, dt, a, b}] = TuneChoice{t, ms, true};
   }
   std::fclose(fp);

@@ -297,6 +297,12 @@ int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::Lau
   if (n <= 1) return PX_OK;
   ++g_stats.sort;
   ++g_stats.launches;
+  if (pxk::radix_sort_ok(dtype, n)) {
+    if (!c.reserve(pxk::radix_status_words(dtype, n), n * pxk::dtype_size(dtype))) return PX_ERR_NOMEM;
+    for (int i = 0; i < pxk::radix_passes(dtype); ++i)
+      if (!c.note_status_launch()) return PX_ERR_DEVICE;
+    return dev_err(pxk::radix_sort(dtype, data, n, c.scratch(), c.stream(), hook), "sort");
+  }
   return dev_err(pxk::sort(dtype, data, n, c.stream(), hook), "sort");
 }
 
@@ -381,7 +387,7 @@ std::vector<pxk::TileCfg> seg_tile_candidates(int op, int dtype) {
   } else {
     for (pxk::TileCfg t : {pxk::TileCfg{256, 4}, pxk::TileCfg{256, 8}, pxk::TileCfg{512, 8}, pxk::TileCfg{1024, 4},
                            pxk::TileCfg{1024, 8}})
-      if (pxk::tile_cfg_supported(t) && !(dtype == PX_F64 && t.block == 1024 && t.k == 8)) v.push_back(t);
+      if (pxk::tile_cfg_supported(t, op) && !(dtype == PX_F64 && t.block == 1024 && t.k == 8)) v.push_back(t);
   }
   return v;
 }

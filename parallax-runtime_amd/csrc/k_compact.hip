@@ -156,6 +156,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     C tile_kept = 0;
 #pragma unroll
     for (int j = 0; j < NW; ++j) tile_kept += s_w[j];
+    // publish the tile's count before staging, so later tiles' look-backs need not wait
+    if (threadIdx.x == 0 && tile != ti.first_tile) publish<C>(status, tile, epoch, kStAgg, tile_kept);
     const int vstart_r = lo_r > 0 ? lo_r : 0;  // first valid offset of the window
     const C tile_valid = static_cast<C>(hi_r - vstart_r);
 
@@ -185,7 +187,6 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
       if (tile == ti.first_tile) {
         prefix = 0;
       } else {
-        if (lane == 0) publish<C>(status, tile, epoch, kStAgg, tile_kept);
         prefix = lookback<C>(status, tile, ti.first_tile, epoch);
       }
       if (tile != ti.last_tile && lane == 0)

@@ -31,6 +31,13 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
                                               Ctl* ctl) {
   using A = typename Traits<T>::A;
   constexpr int VEC = 16 / sizeof(T);
+  // A ticket covers S sub-tiles of KR register chunks each; all but the last sub-tile
+  // wait in LDS (lane-private slots) while the last is scanned in registers, so one
+  // look-back serves S * KR chunks without raising register pressure.
+  constexpr int KR = K > 8 ? 8 : K;
+  constexpr int S = K / KR;
+  static_assert(KR * S == K, "tile chunks must be a multiple of the register chunks");
+  constexpr int SUB = BLOCK * KR * VEC;
   constexpr int TILE = BLOCK * K * VEC;
   constexpr int NW = BLOCK / kWave;
   using V = Vec<T, VEC>;
@@ -39,18 +46,20 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
-  __shared__ A s_w[NW];
+  __shared__ A s_w[S][NW];
   __shared__ A s_prefix;
+  __shared__ V s_stash[S > 1 ? (S - 1) * KR * BLOCK : 1];
 
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
 
-  // Masked tile load (zeros outside the tile's segment window).
-  auto load = [&](T (&dst)[K][VEC], const TileInfo& t) {
-    if (t.w0 >= t.lo && t.w0 + TILE <= t.hi) {  // whole tile inside: no masks
-      const V* tp = vin + t.w0 / VEC + wid * (K * kWave) + lane;
+  // Masked load of sub-tile `sub` (zeros outside the tile's segment window).
+  auto load = [&](T (&dst)[KR][VEC], const TileInfo& t, int sub) {
+    const int64_t b0 = t.w0 + static_cast<int64_t>(sub) * SUB;
+    if (b0 >= t.lo && b0 + SUB <= t.hi) {  // whole sub-tile inside: no masks
+      const V* tp = vin + b0 / VEC + wid * (KR * kWave) + lane;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
+      for (int k = 0; k < KR; ++k) {
         const V v = ld_stream(tp + k * kWave);
 #pragma unroll
         for (int j = 0; j < VEC; ++j) dst[k][j] = v.v[j];
@@ -58,8 +67,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
       return;
     }
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t p0 = t.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
+    for (int k = 0; k < KR; ++k) {
+      const int64_t p0 = b0 + ((static_cast<int64_t>(wid) * KR + k) * kWave + lane) * VEC;
       if (p0 >= t.lo && p0 + VEC <= t.hi) {
         const V v = vin[p0 / VEC];
 #pragma unroll
@@ -71,6 +80,61 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
       } else {
 #pragma unroll
         for (int j = 0; j < VEC; ++j) dst[k][j] = T(0);
+      }
+    }
+  };
+
+  // Per-chunk wave scans carried across the wave's chunks; returns the wave total.
+  auto local_scan = [&](const T (&x)[KR][VEC], A (&tex)[KR]) {
+    A run = acc_zero<A>();
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      A c = to_acc(x[k][0]);
+#pragma unroll
+      for (int j = 1; j < VEC; ++j) c = acc_add(c, to_acc(x[k][j]));
+      const A incl = wave_incl_scan(c);
+      const A tot = read_lane(incl, kWave - 1);
+      tex[k] = acc_add(run, wave_shift_right1(incl));
+      run = acc_add(run, tot);
+    }
+    return run;
+  };
+
+  // Outputs of one sub-tile (16-byte stores when the output lines up with the input
+  // vectors); `base` is the carry into this wave's first chunk.
+  auto emit = [&](const T (&x)[KR][VEC], const A (&tex)[KR], A base, const TileInfo& ti, int sub) {
+    const int64_t odelta = ti.out_delta - in_shift;  // output element = position + odelta
+    const bool vec_ok = ((out_shift + odelta) & (VEC - 1)) == 0;
+    const int64_t b0 = ti.w0 + static_cast<int64_t>(sub) * SUB;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      A r = acc_add(base, tex[k]);
+      T xv[VEC];
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        xv[j] = x[k][j];
+        // Widening carries: convert again here rather than keep the widened values of
+        // the whole tile alive from the local scan (twice the registers).
+        if constexpr (sizeof(A) > sizeof(T)) asm volatile("" : "+v"(xv[j]));
+      }
+      V y;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        if (EXCL) {
+          y.v[j] = from_acc<T>(r);
+          r = acc_add(r, to_acc(xv[j]));
+        } else {
+          r = acc_add(r, to_acc(xv[j]));
+          y.v[j] = from_acc<T>(r);
+        }
+      }
+      const int64_t p0 = b0 + ((static_cast<int64_t>(wid) * KR + k) * kWave + lane) * VEC;
+      if (vec_ok && p0 >= ti.lo && p0 + VEC <= ti.hi) {
+        st_stream(&vout[(p0 + odelta + out_shift) / VEC], y);
+      } else if (p0 < ti.hi && p0 + VEC > ti.lo) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          if (p0 + j >= ti.lo && p0 + j < ti.hi) out[p0 + j + odelta] = y.v[j];
       }
     }
   };
@@ -87,33 +151,40 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     const uint32_t epoch = s_epoch;
     if (tile >= g.ntiles) break;
     const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
-    T x[K][VEC];
-    load(x, ti);
+    T x[KR][VEC];
 
-    // ---- per-chunk wave scans, carried across the wave's chunks
-    A run = acc_zero<A>();
-    A tex[K];
+    // ---- leading sub-tiles: wave totals only, values parked in LDS
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      A c = to_acc(x[k][0]);
+    for (int sub = 0; sub < S - 1; ++sub) {
+      load(x, ti, sub);
+      A c = acc_zero<A>();
 #pragma unroll
-      for (int j = 1; j < VEC; ++j) c = acc_add(c, to_acc(x[k][j]));
-      const A incl = wave_incl_scan(c);
-      const A tot = read_lane(incl, kWave - 1);
-      tex[k] = acc_add(run, wave_shift_right1(incl));
-      run = acc_add(run, tot);
+      for (int k = 0; k < KR; ++k) {
+        V v;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          v.v[j] = x[k][j];
+          c = acc_add(c, to_acc(x[k][j]));
+        }
+        s_stash[(sub * KR + k) * BLOCK + threadIdx.x] = v;
+      }
+      const A tot = read_lane(wave_incl_scan(c), kWave - 1);
+      if (lane == 0) s_w[sub][wid] = tot;
     }
-    if (lane == 0) s_w[wid] = run;
+
+    // ---- last sub-tile scanned in registers
+    load(x, ti, S - 1);
+    A tex[KR];
+    const A run = local_scan(x, tex);
+    if (lane == 0) s_w[S - 1][wid] = run;
     __syncthreads();
 
-    // ---- wave offsets and tile total in a fixed order
-    A wave_off = acc_zero<A>();
-    for (int j = 0; j < wid; ++j) wave_off = acc_add(wave_off, s_w[j]);
-
     if (wid == 0) {
-      A tile_tot = s_w[0];
+      A tile_tot = s_w[0][0];
 #pragma unroll
-      for (int j = 1; j < NW; ++j) tile_tot = acc_add(tile_tot, s_w[j]);
+      for (int sub = 0; sub < S; ++sub)
+#pragma unroll
+        for (int j = (sub == 0 ? 1 : 0); j < NW; ++j) tile_tot = acc_add(tile_tot, s_w[sub][j]);
       A prefix;
       if (tile == ti.first_tile) {
         prefix = EXCL ? init : acc_zero<A>();
@@ -127,32 +198,34 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     }
     __syncthreads();
 
-    // ---- outputs (16-byte stores when the output lines up with the input vectors)
-    const A base = acc_add(s_prefix, wave_off);
-    const int64_t odelta = ti.out_delta - in_shift;  // output element = position + odelta
-    const bool vec_ok = ((out_shift + odelta) & (VEC - 1)) == 0;
+    // ---- carries into each sub-tile and wave, in a fixed order
+    A sub_base[S];
+    A wave_off[S];
+    {
+      A b = s_prefix;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      A r = acc_add(base, tex[k]);
-      V y;
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        if (EXCL) {
-          y.v[j] = from_acc<T>(r);
-          r = acc_add(r, to_acc(x[k][j]));
-        } else {
-          r = acc_add(r, to_acc(x[k][j]));
-          y.v[j] = from_acc<T>(r);
-        }
+      for (int sub = 0; sub < S; ++sub) {
+        sub_base[sub] = b;
+        A w = acc_zero<A>();
+        for (int j = 0; j < wid; ++j) w = acc_add(w, s_w[sub][j]);
+        wave_off[sub] = w;
+        if (sub + 1 < S)
+          for (int j = 0; j < NW; ++j) b = acc_add(b, s_w[sub][j]);
       }
-      const int64_t p0 = ti.w0 + ((static_cast<int64_t>(wid) * K + k) * kWave + lane) * VEC;
-      if (vec_ok && p0 >= ti.lo && p0 + VEC <= ti.hi) {
-        st_stream(&vout[(p0 + odelta + out_shift) / VEC], y);
-      } else if (p0 < ti.hi && p0 + VEC > ti.lo) {
+    }
+    emit(x, tex, acc_add(sub_base[S - 1], wave_off[S - 1]), ti, S - 1);
+
+    // ---- parked sub-tiles: rescan from LDS (each lane reads back its own slots)
 #pragma unroll
-        for (int j = 0; j < VEC; ++j)
-          if (p0 + j >= ti.lo && p0 + j < ti.hi) out[p0 + j + odelta] = y.v[j];
+    for (int sub = 0; sub < S - 1; ++sub) {
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        const V v = s_stash[(sub * KR + k) * BLOCK + threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) x[k][j] = v.v[j];
       }
+      local_scan(x, tex);
+      emit(x, tex, acc_add(sub_base[sub], wave_off[sub]), ti, sub);
     }
     __syncthreads();
   }
@@ -197,6 +270,7 @@ hipError_t scan_t(const void* in, void* out, const SegGeom& g, bool exclusive, c
   PXK_SCAN(512, 8)
   PXK_SCAN(1024, 4)
   PXK_SCAN(1024, 8)
+  PXK_SCAN(1024, 16)
 #undef PXK_SCAN
   return hipErrorInvalidValue;
 }

@@ -26,6 +26,8 @@ struct Ctl {
   uint64_t result;  // kept count written by the last compaction tile
   uint64_t pad1[5];
   uint8_t value[64];  // reduce result slot (one element, written by the last block)
+  uint32_t radix_hist[8 * 256];  // radix sort digit counts (zero between calls)
+  uint32_t radix_bins[8 * 256];  // radix sort exclusive bin starts per digit position
 };
 
 constexpr uint32_t kEpochMax = (1u << 30) - 1;
@@ -52,9 +54,12 @@ struct Tune {
   int red_nt = 1;             // non-temporal loads in the plain reduce
   int sort_c4 = 6;            // stages per HBM register pass of the sort, 4-byte keys (<= 7)
   int sort_c8 = 6;            // ... 8-byte keys (<= 6)
+  uint64_t sort_radix_min = 1ull << 20;  // plain sorts of at least this many keys use radix (0: never)
 };
 Tune& tune();
-bool tile_cfg_supported(TileCfg c);
+// Compiled tile shapes of the look-back kernels: op PX_OP_SCAN (scans) or PX_OP_FLAGS
+// (compaction); scans also have 1024 x 16 (two register sub-tiles per ticket).
+bool tile_cfg_supported(TileCfg c, int op);
 // Forced tile shape of (op, dtype) for plain and batched calls (op: PX_OP_REDUCE
 // (batched only), PX_OP_SCAN, PX_OP_FLAGS); {0, 0} when none is set.
 TileCfg tile_override(int op, int dtype);
@@ -147,6 +152,15 @@ struct LaunchHook {
 // "sort_tile" (LDS sort of each tile), "sort_global" (register pass over HBM) and
 // "sort_merge" (LDS half-cleaners of one level); each reads and writes the array once.
 hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream, const LaunchHook& hook = {});
+// ---- radix sort (k_radix.hip): same ascending order for large arrays; kernels
+// "sort_hist" (digit histograms, one read) and "sort_radix" (one per 8-bit digit, one
+// read + write each).  Scratch: radix_status_words() status words and n * size bytes of
+// partials (the second buffer).
+bool radix_sort_ok(int dtype, uint64_t n);
+uint64_t radix_status_words(int dtype, uint64_t n);
+int radix_passes(int dtype);
+hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipStream_t stream,
+                      const LaunchHook& hook = {});
 // Segmented sort of [seg_off[s], seg_off[s+1]) for every segment: a host plan (block
 // tables for each launch kind) uploaded by the caller, then a fixed launch sequence.
 struct SegSortPlan {

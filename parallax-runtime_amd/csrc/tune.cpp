@@ -7,9 +7,9 @@
 
 namespace pxk {
 
-bool tile_cfg_supported(TileCfg c) {
+bool tile_cfg_supported(TileCfg c, int op) {
   return (c.block == 256 && (c.k == 4 || c.k == 8)) || (c.block == 512 && c.k == 8) ||
-         (c.block == 1024 && (c.k == 4 || c.k == 8));
+         (c.block == 1024 && (c.k == 4 || c.k == 8 || (op == PX_OP_SCAN && c.k == 16)));
 }
 
 namespace {
@@ -30,7 +30,7 @@ bool set_tile_override(int op, int dtype, TileCfg c) {
   if (i < 0 || dtype < 0 || dtype >= PX_NUM_DTYPES) return false;
   const bool clear = c.block == 0 && c.k == 0;
   const bool ok = clear || (op == PX_OP_REDUCE ? (c.block == 256 && (c.k == 4 || c.k == 8 || c.k == 16))
-                                               : tile_cfg_supported(c));
+                                               : tile_cfg_supported(c, op));
   if (!ok) return false;
   g_override[i][dtype] = c;
   return true;
@@ -81,12 +81,13 @@ Tune load() {
     else if (k == "red_nt") t.red_nt = v;
     else if (k == "sort_c4" && v >= 1 && v <= 7) t.sort_c4 = v;
     else if (k == "sort_c8" && v >= 1 && v <= 6) t.sort_c8 = v;
+    else if (k == "sort_radix_min" && v >= 0) t.sort_radix_min = static_cast<uint64_t>(v);
   }
   for (int d = 0; d < PX_NUM_DTYPES; ++d) {
     TileCfg a{scan.block ? scan.block : t.scan[d].block, scan.k ? scan.k : t.scan[d].k};
-    if (tile_cfg_supported(a)) t.scan[d] = a;
+    if (tile_cfg_supported(a, PX_OP_SCAN)) t.scan[d] = a;
     TileCfg b{compact.block ? compact.block : t.compact[d].block, compact.k ? compact.k : t.compact[d].k};
-    if (tile_cfg_supported(b)) t.compact[d] = b;
+    if (tile_cfg_supported(b, PX_OP_FLAGS)) t.compact[d] = b;
   }
   return t;
 }

@@ -153,6 +153,77 @@ def test_sort_extremes(gpu_ctx):
     assert np.array_equal(b.download(np.uint32, u.size), np.sort(u))
 
 
+
+RADIX_SIZES = [1 << 20, (1 << 20) + 3, 3 * (1 << 20) + 7, (1 << 22) + 1]
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", RADIX_SIZES)
+def test_sort_radix(gpu_ctx, dt, n):
+    """Arrays at or above the radix threshold (full range of the type's values)."""
+    rng = np.random.default_rng(900 + n % 97 + dt)
+    if dt in FLOATS:
+        a = (rng.standard_normal(n) * np.exp(rng.uniform(-30, 30, n))).astype(NP_DTYPES[dt])
+    else:
+        info = np.iinfo(NP_DTYPES[dt])
+        a = rng.integers(info.min, info.max, n, dtype=NP_DTYPES[dt], endpoint=True)
+    a[::5] = a[3]  # duplicates
+    shift = n % 2
+    b, p = upload(a, shift)
+    check(lib.pxd_sort(gpu_ctx.h, dt, p, n), "sort")
+    gpu_ctx.sync()
+    got = b.download(a.dtype, n, shift * a.itemsize)
+    assert got.tobytes() == pxo.sort(a).tobytes()
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_sort_radix_patterns(gpu_ctx, dt):
+    """All-equal, sorted, reversed and extreme keys through the radix path."""
+    n = (1 << 20) + 5
+    np_t = NP_DTYPES[dt]
+    if dt in FLOATS:
+        fi = np.finfo(np_t)
+        ext = np.array([np.inf, -np.inf, fi.max, -fi.max, fi.tiny, -fi.tiny, fi.tiny / 4, 0.0, 1.0, -1.0], np_t)
+    else:
+        ii = np.iinfo(np_t)
+        ext = np.array([ii.min, ii.max, 0, 1, ii.max - 1, ii.min + 1] + ([-1] if ii.min < 0 else []), np_t)
+    base = gen(dt, n, 77 + dt)
+    cases = [np.full(n, base[0]), np.sort(base), np.sort(base)[::-1].copy(), np.resize(ext, n)]
+    for a in cases:
+        a = np.ascontiguousarray(a, dtype=np_t)
+        b, p = upload(a)
+        check(lib.pxd_sort(gpu_ctx.h, dt, p, n), "sort")
+        gpu_ctx.sync()
+        got = b.download(np_t, n)
+        assert got.tobytes() == pxo.sort(a).tobytes()
+
+
+def test_sort_radix_signed_zero_nan(gpu_ctx):
+    """-0.0 / +0.0 are equal under operator<, NaNs unordered: the radix order puts -0.0
+    before +0.0 and NaNs by sign at the ends; everything else matches std::sort."""
+    n = (1 << 20) + 1
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal(n).astype(np.float32)
+    a[::3] = 0.0
+    a[1::3] = -0.0
+    a[7::1001] = np.nan
+    a[9::1003] = -np.nan
+    b, p = upload(a)
+    check(lib.pxd_sort(gpu_ctx.h, PX_F32, p, n), "sort")
+    gpu_ctx.sync()
+    got = b.download(np.float32, n)
+    bits = got.view(np.uint32)
+    neg_nan = np.isnan(got) & (bits >> 31 == 1)
+    pos_nan = np.isnan(got) & (bits >> 31 == 0)
+    k_neg, k_pos = int(neg_nan.sum()), int(pos_nan.sum())
+    assert neg_nan[:k_neg].all() and pos_nan[n - k_pos:].all()
+    mid = got[k_neg:n - k_pos]
+    ref = np.sort(a[~np.isnan(a)])
+    assert np.array_equal(mid, ref)  # equal values (-0.0 == +0.0)
+    z = mid[mid == 0].view(np.uint32)
+    assert (np.diff((z >> 31) ^ 1) >= 0).all()  # every -0.0 before every +0.0
+    assert sorted(got.view(np.uint32).tolist()) == sorted(a.view(np.uint32).tolist())  # permutation
+
 PREDS = [
     (C.PRED_GT, 0, 0, 0), (C.PRED_LT, 100, 0, C.FLAG_NEGATE), (C.PRED_MOD_EQ, 2, 0, 0),
     (C.PRED_IN_RANGE, -1000, 1000, 0), (C.PRED_NONZERO, 0, 0, 0), (C.PRED_TRUE, 0, 0, 0),
