@@ -36,9 +36,7 @@ namespace pxk {
 
 namespace {
 
-constexpr int kRadixBlock = 1024;
 constexpr int kBins = 256;
-constexpr int kHistBlock = 256;
 constexpr int kLookWindow = 4;  // predecessor tiles read per look-back round trip
 
 template <class T>
@@ -87,32 +85,35 @@ __device__ __forceinline__ uint32_t digit_of(T x, int shift) {
 
 template <class T>
 constexpr int kKpt = sizeof(T) == 4 ? 16 : 8;  // keys per lane
-template <class T>
-constexpr int kRadixTile = kRadixBlock * kKpt<T>;
 
 __device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t state, uint32_t v) {
   return (static_cast<uint64_t>((epoch << 2) | state) << 32) | v;
 }
 
 // ---------------------------------------------------------------- histograms
-template <class T>
-__global__ __launch_bounds__(kHistBlock) void k_radix_hist(const T* data, uint64_t n, Ctl* ctl) {
+// Each wave counts into its own LDS histograms (no contention between waves); the
+// block merges them and adds the totals to the control block's counters.
+template <class T, int HB>
+__global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ctl* ctl) {
   constexpr int D = sizeof(T);  // 8-bit digits per key
   constexpr int VEC = 16 / sizeof(T);
+  constexpr int NWH = HB / kWave;
   using V = Vec<T, VEC>;
-  __shared__ uint32_t h[D * kBins];
-  __shared__ uint32_t s_w[kHistBlock / kWave];
+  __shared__ uint32_t h[NWH][D * kBins];
+  __shared__ uint32_t s_w[kBins / kWave];
   __shared__ int s_last;
 
-  for (int i = threadIdx.x; i < D * kBins; i += kHistBlock) h[i] = 0;
-  __syncthreads();
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  uint32_t* hw = h[wid];
+  for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0;  // the wave's own copy
   auto count = [&](T x) {
     const auto k = RadixKey<T>::enc(x);
 #pragma unroll
-    for (int d = 0; d < D; ++d) atomicAdd(&h[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
+    for (int d = 0; d < D; ++d) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
   };
-  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * kHistBlock + threadIdx.x;
-  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * kHistBlock;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * HB + threadIdx.x;
+  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * HB;
   const uint32_t mis = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(data) & 15u) / sizeof(T));
   const uint32_t head_v = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
   const uint64_t head = head_v < static_cast<uint32_t>(n) ? head_v : static_cast<uint32_t>(n);  // n < 2^32
@@ -127,8 +128,12 @@ __global__ __launch_bounds__(kHistBlock) void k_radix_hist(const T* data, uint64
   const uint64_t body_end = head + nv * VEC;
   if (gtid < n - body_end) count(data[body_end + gtid]);
   __syncthreads();
-  for (int i = threadIdx.x; i < D * kBins; i += kHistBlock)
-    if (h[i]) __hip_atomic_fetch_add(&ctl->radix_hist[i], h[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = threadIdx.x; i < D * kBins; i += HB) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NWH; ++w) c += h[w][i];
+    if (c) __hip_atomic_fetch_add(&ctl->radix_hist[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -139,32 +144,38 @@ __global__ __launch_bounds__(kHistBlock) void k_radix_hist(const T* data, uint64
   if (!s_last) return;
   __threadfence();
 
-  // Last block: exclusive bin starts of every digit position; counters cleared.
-  const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
+  // Last block: exclusive bin starts of every digit position (one thread per bin);
+  // counters cleared.
   for (int d = 0; d < D; ++d) {
-    const uint32_t c = ld_agent32(&ctl->radix_hist[d * kBins + threadIdx.x]);
-    const uint32_t incl = wave_incl_scan(c);
-    if (lane == kWave - 1) s_w[wid] = incl;
+    uint32_t c = 0, incl = 0;
+    if (threadIdx.x < kBins) {
+      c = ld_agent32(&ctl->radix_hist[d * kBins + threadIdx.x]);
+      incl = wave_incl_scan(c);
+      if (lane == kWave - 1) s_w[wid] = incl;
+    }
     __syncthreads();
-    uint32_t off = incl - c;
-    for (int w = 0; w < wid; ++w) off += s_w[w];
-    st_agent32(&ctl->radix_bins[d * kBins + threadIdx.x], off);
-    st_agent32(&ctl->radix_hist[d * kBins + threadIdx.x], 0u);
+    if (threadIdx.x < kBins) {
+      uint32_t off = incl - c;
+      for (int w = 0; w < wid; ++w) off += s_w[w];
+      st_agent32(&ctl->radix_bins[d * kBins + threadIdx.x], off);
+      st_agent32(&ctl->radix_hist[d * kBins + threadIdx.x], 0u);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) st_agent32(&ctl->done, 0u);
 }
 
 // ---------------------------------------------------------------- one digit pass
-template <class T>
-__global__ __launch_bounds__(kRadixBlock) void k_radix_pass(const T* __restrict__ in,
-                                                          T* __restrict__ out, uint64_t n,
-                                                          int shift, uint64_t* status, Ctl* ctl) {
+// Four waves per SIMD (128 VGPRs): 1024 threads per CU whatever the block size.
+template <class T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_radix_pass(const T* __restrict__ in,
+                                                    T* __restrict__ out, uint64_t n,
+                                                    int shift, uint64_t* status, Ctl* ctl) {
+  static_assert(BLOCK >= kBins, "one thread per digit in the count phases");
   constexpr int KPT = kKpt<T>;
-  constexpr int NW = kRadixBlock / kWave;
+  constexpr int NW = BLOCK / kWave;
   constexpr int WSPAN = KPT * kWave;
-  constexpr int TILE = kRadixTile<T>;
+  constexpr int TILE = BLOCK * KPT;
   constexpr int R = kLookWindow;
   const uint32_t ntiles = static_cast<uint32_t>((n + TILE - 1) / TILE);
 
@@ -210,13 +221,25 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_pass(const T* __restrict_
     for (int j = 0; j < KPT; ++j) {
       const bool valid = wbase + static_cast<uint64_t>(j) * kWave + lane < n;
       const uint32_t d = digit_of(key[j], shift);
-      uint64_t peers = __ballot(valid);
+      // Lanes whose digit differs from this lane's: some bit's ballot disagrees with
+      // this lane's bit (sign-extended to a full word, the same for both halves).
+      uint32_t own[8];
+      uint64_t m[8];
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+        own[b] = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int32_t>(d), b, 1));
+        asm volatile("" : "+v"(own[b]));  // ballot from the extracted bit: one compare
+        m[b] = __ballot(own[b] != 0u);
       }
+      // diff |= m ^ own in one three-input op per half (truth table 0xF6 for
+      // S0 | (S1 ^ S2) with S0, S1, S2 = 0xF0, 0xCC, 0xAA).
+      uint32_t diff_lo = 0, diff_hi = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, static_cast<uint32_t>(m[b]), own[b], 0xF6);
+        diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, static_cast<uint32_t>(m[b] >> 32), own[b], 0xF6);
+      }
+      const uint64_t peers = __ballot(valid) & ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
       const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below));
       const uint32_t c = s_wh[wid][d];
       rank[j] = c + below;
@@ -241,11 +264,28 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_pass(const T* __restrict_
     }
     __syncthreads();
 
+    // ---- tile-local starts of every (wave, digit)
+    uint32_t lstart = 0;
+    if (threadIdx.x < kBins) {
+      const int b = threadIdx.x;
+      lstart = incl - tot;
+      for (int w = 0; w < wid; ++w) lstart += s_wsum[w];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) s_wh[w][b] += lstart;
+    }
+    __syncthreads();
+
+    // ---- reorder the tile in LDS by (digit, rank); it needs no global offsets, so it
+    // runs before the look-back and gives the predecessors time to publish
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const bool valid = wbase + static_cast<uint64_t>(j) * kWave + lane < n;
+      if (valid) s_keys[s_wh[wid][digit_of(key[j], shift)] + rank[j]] = key[j];
+    }
+
     // ---- digit counts before this tile (windowed look-back), output offsets
     if (threadIdx.x < kBins) {
       const int b = threadIdx.x;
-      uint32_t lstart = incl - tot;
-      for (int w = 0; w < wid; ++w) lstart += s_wsum[w];
       uint32_t prefix = 0;
       if (tile > 0) {
         int64_t t = static_cast<int64_t>(tile) - 1;
@@ -277,16 +317,6 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_pass(const T* __restrict_
                  status_word(epoch, kStIncl, prefix + tot));
       }
       s_goff[b] = bins[b] + prefix - lstart;  // modulo 2^32; n < 2^32
-#pragma unroll
-      for (int w = 0; w < NW; ++w) s_wh[w][b] += lstart;
-    }
-    __syncthreads();
-
-    // ---- reorder the tile in LDS by (digit, rank)
-#pragma unroll
-    for (int j = 0; j < KPT; ++j) {
-      const bool valid = wbase + static_cast<uint64_t>(j) * kWave + lane < n;
-      if (valid) s_keys[s_wh[wid][digit_of(key[j], shift)] + rank[j]] = key[j];
     }
     __syncthreads();
 
@@ -299,12 +329,12 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_pass(const T* __restrict_
     if (tn == TILE) {
 #pragma unroll
       for (int i0 = 0; i0 < KPT; ++i0) {
-        const uint32_t i = static_cast<uint32_t>(i0) * kRadixBlock + threadIdx.x;
+        const uint32_t i = static_cast<uint32_t>(i0) * BLOCK + threadIdx.x;
         const T k = s_keys[i];
         out[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
       }
     } else {
-      for (uint32_t i = threadIdx.x; i < tn; i += kRadixBlock) {
+      for (uint32_t i = threadIdx.x; i < tn; i += BLOCK) {
         const T k = s_keys[i];
         out[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
       }
@@ -314,37 +344,60 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_pass(const T* __restrict_
   if (threadIdx.x == 0) block_end(ctl, s_epoch);
 }
 
-template <class T>
-hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t stream,
-                        const LaunchHook& hook) {
-  T* data = static_cast<T*>(data_v);
-  T* tmp = static_cast<T*>(s.partials);
-  constexpr uint64_t TILE = kRadixTile<T>;
-  const uint64_t ntiles = (n + TILE - 1) / TILE;
-  if (n >= (1ull << 32) || ntiles * kBins > s.status_words || s.partial_bytes < n * sizeof(T) || !tmp)
-    return hipErrorInvalidValue;
-  const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
-  const uint64_t hist_work = (n + kHistBlock * 64 - 1) / (kHistBlock * 64);
-  const uint32_t hist_grid = static_cast<uint32_t>(std::min<uint64_t>(2ull * cus, hist_work ? hist_work : 1));
-  hook("sort_hist", 0);
-  k_radix_hist<T><<<hist_grid, kHistBlock, 0, stream>>>(data, n, s.ctl);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hook("sort_hist", 1);
-  // One block per CU: every block of the grid is resident (a look-back waits only on
-  // running blocks).
-  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ntiles, cus));
+// Threads per digit-pass block (256, 512 or 1024); blocks per CU = 1024 / block.
+int radix_block() {
+  const int b = tune().radix_block;
+  return (b == 256 || b == 512 || b == 1024) ? b : 1024;
+}
+
+uint64_t radix_tile(size_t elem) { return static_cast<uint64_t>(radix_block()) * (elem == 4 ? kKpt<uint32_t> : kKpt<uint64_t>); }
+
+template <class T, int BLOCK>
+hipError_t radix_passes_t(T* data, T* tmp, uint64_t n, uint32_t grid, const Scratch& s,
+                          hipStream_t stream, const LaunchHook& hook) {
   T* a = data;
   T* b = tmp;
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
     hook("sort_radix", 0);
-    k_radix_pass<T><<<grid, kRadixBlock, 0, stream>>>(a, b, n, 8 * d, s.status, s.ctl);
-    e = hipGetLastError();
+    k_radix_pass<T, BLOCK><<<grid, BLOCK, 0, stream>>>(a, b, n, 8 * d, s.status, s.ctl);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hook("sort_radix", 1);
     std::swap(a, b);
   }
   return hipSuccess;
+}
+
+template <class T>
+hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t stream,
+                        const LaunchHook& hook) {
+  T* data = static_cast<T*>(data_v);
+  T* tmp = static_cast<T*>(s.partials);
+  const int block = radix_block();
+  const uint64_t TILE = radix_tile(sizeof(T));
+  const uint64_t ntiles = (n + TILE - 1) / TILE;
+  if (n >= (1ull << 32) || ntiles * kBins > s.status_words || s.partial_bytes < n * sizeof(T) || !tmp)
+    return hipErrorInvalidValue;
+  const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
+  // Histograms: 256-thread blocks, two per CU (measured best of 1, 2 and 4: the LDS
+  // atomics, not the loads, bound this kernel), no more blocks than 16 vectors per
+  // thread need.
+  constexpr int HB = 256;
+  const uint64_t hist_per_cu = tune().radix_hist_bpc > 0 ? static_cast<uint64_t>(tune().radix_hist_bpc) : 2;
+  const uint64_t hist_work = (n + HB * 16ull * (16 / sizeof(T)) - 1) / (HB * 16ull * (16 / sizeof(T)));
+  const uint32_t hist_grid = static_cast<uint32_t>(std::min<uint64_t>(hist_per_cu * cus, hist_work ? hist_work : 1));
+  hook("sort_hist", 0);
+  k_radix_hist<T, HB><<<hist_grid, HB, 0, stream>>>(data, n, s.ctl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hook("sort_hist", 1);
+  // 1024 threads per CU (1024 / block blocks fit one CU's registers and LDS).  Tiles
+  // are taken by ticket when a block runs, so a look-back only ever waits on tiles held
+  // by running blocks, whatever the grid.
+  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ntiles, static_cast<uint64_t>(cus) * (1024 / block)));
+  if (block == 256) return radix_passes_t<T, 256>(data, tmp, n, grid, s, stream, hook);
+  if (block == 1024) return radix_passes_t<T, 1024>(data, tmp, n, grid, s, stream, hook);
+  return radix_passes_t<T, 512>(data, tmp, n, grid, s, stream, hook);
 }
 
 }  // namespace
@@ -359,7 +412,7 @@ bool radix_sort_ok(int dtype, uint64_t n) {
 }
 
 uint64_t radix_status_words(int dtype, uint64_t n) {
-  const uint64_t tile = dtype_size(dtype) == 4 ? kRadixTile<uint32_t> : kRadixTile<uint64_t>;
+  const uint64_t tile = radix_tile(dtype_size(dtype));
   return ((n + tile - 1) / tile) * kBins;
 }
 

@@ -55,6 +55,8 @@ struct Tune {
   int sort_c4 = 6;            // stages per HBM register pass of the sort, 4-byte keys (<= 7)
   int sort_c8 = 6;            // ... 8-byte keys (<= 6)
   uint64_t sort_radix_min = 1ull << 20;  // plain sorts of at least this many keys use radix (0: never)
+  int radix_block = 1024;     // threads per radix digit-pass block (256, 512, 1024)
+  int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 2)
 };
 Tune& tune();
 // Compiled tile shapes of the look-back kernels: op PX_OP_SCAN (scans) or PX_OP_FLAGS
