@@ -78,6 +78,19 @@ struct RadixKey<double> {
   }
 };
 
+// The key whose order-preserving bits are all ones (every digit 255): pads positions
+// past n, which then sort after every real key of their tile.
+template <class T>
+__device__ __forceinline__ T radix_pad() {
+  if constexpr (sizeof(T) == 4) {
+    const uint32_t u = std::is_same_v<T, uint32_t> ? 0xffffffffu : 0x7fffffffu;
+    return __builtin_bit_cast(T, u);
+  } else {
+    const uint64_t u = std::is_same_v<T, uint64_t> ? ~0ull : 0x7fffffffffffffffull;
+    return __builtin_bit_cast(T, u);
+  }
+}
+
 template <class T>
 __device__ __forceinline__ uint32_t digit_of(T x, int shift) {
   return static_cast<uint32_t>(RadixKey<T>::enc(x) >> shift) & 0xffu;
@@ -208,23 +221,28 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t tile = s_tile;
     const uint32_t epoch = s_epoch;
     if (tile >= ntiles) break;
-    const uint64_t tbase = static_cast<uint64_t>(tile) * TILE;
-    const uint64_t wbase = tbase + static_cast<uint64_t>(wid) * WSPAN;
+    // Positions in 32-bit arithmetic (n < 2^32): key j of this lane is real when j < nv.
+    // Positions past n hold the pad key: digit 255 in every pass and last in input
+    // order, so they take the tile's last local indices [tn, TILE), which are never
+    // written out (only the last tile has them, and nothing reads its counts).
+    const uint32_t n32 = static_cast<uint32_t>(n);
+    const uint32_t tbase = tile * static_cast<uint32_t>(TILE);
+    const uint32_t wbase = tbase + static_cast<uint32_t>(wid * WSPAN);
+    const uint32_t first = wbase + static_cast<uint32_t>(lane);
+    const uint32_t left = first < n32 ? (n32 - first + kWave - 1) / kWave : 0u;
+    const int nv = left < static_cast<uint32_t>(KPT) ? static_cast<int>(left) : KPT;
 
     // ---- loads (a wave's key j of all lanes is 64 consecutive keys)
     T key[KPT];
+    const T* lp = in + first;
 #pragma unroll
-    for (int j = 0; j < KPT; ++j) {
-      const uint64_t p = wbase + static_cast<uint64_t>(j) * kWave + lane;
-      key[j] = p < n ? __builtin_nontemporal_load(in + p) : T(0);
-    }
+    for (int j = 0; j < KPT; ++j) key[j] = j < nv ? __builtin_nontemporal_load(lp + j * kWave) : radix_pad<T>();
 
     // ---- rank among equal digits of the wave, in input order (two 16-bit ranks per
     // register: a tile holds fewer than 2^16 keys)
     uint32_t rank2[(KPT + 1) / 2] = {};
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
-      const bool valid = wbase + static_cast<uint64_t>(j) * kWave + lane < n;
       const uint32_t d = digit_of(key[j], shift);
       // Lanes whose digit differs from this lane's: some bit's ballot disagrees with
       // this lane's bit (sign-extended to a full word, the same for both halves).
@@ -244,11 +262,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, static_cast<uint32_t>(m[b]), own[b], 0xF6);
         diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, static_cast<uint32_t>(m[b] >> 32), own[b], 0xF6);
       }
-      const uint64_t peers = __ballot(valid) & ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
+      const uint64_t peers = ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
       const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below));
       const uint32_t c = s_wh[wid][d];
       rank2[j / 2] |= (c + below) << (16 * (j & 1));
-      if (valid && below == 0) s_wh[wid][d] = c + static_cast<uint32_t>(__popcll(peers));
+      if (below == 0) s_wh[wid][d] = c + static_cast<uint32_t>(__popcll(peers));
     }
     __syncthreads();
 
@@ -284,8 +302,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // runs before the look-back and gives the predecessors time to publish
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
-      const bool valid = wbase + static_cast<uint64_t>(j) * kWave + lane < n;
-      if (valid) s_keys[s_wh[wid][digit_of(key[j], shift)] + ((rank2[j / 2] >> (16 * (j & 1))) & 0xffffu)] = key[j];
+      s_keys[s_wh[wid][digit_of(key[j], shift)] + ((rank2[j / 2] >> (16 * (j & 1))) & 0xffffu)] = key[j];
     }
 
     // ---- digit counts before this tile (windowed look-back), output offsets
@@ -329,8 +346,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // Keys of this tile, in 32-bit arithmetic (n < 2^32).  hipcc lowered the 64-bit form
     // of this select (a vector compare, then a scalar select on SCC) to a select on the
     // borrow of the subtraction, which gave the full tile length for a partial last tile.
-    const uint32_t left = static_cast<uint32_t>(n) - tile * static_cast<uint32_t>(TILE);
-    const uint32_t tn = left < static_cast<uint32_t>(TILE) ? left : static_cast<uint32_t>(TILE);
+    const uint32_t tleft = n32 - tbase;
+    const uint32_t tn = tleft < static_cast<uint32_t>(TILE) ? tleft : static_cast<uint32_t>(TILE);
     if (tn == TILE) {
 #pragma unroll
       for (int i0 = 0; i0 < KPT; ++i0) {
