@@ -464,23 +464,76 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
+  // Segments longer than one bitonic LDS tile go to the segmented radix sort (when the
+  // radix path is enabled and every index fits 32 bits); the rest stay on the bitonic
+  // path, which sorts each in one launch.
+  const uint64_t long_min = pxk::seg_radix_min_len(dtype);
+  const uint32_t rtile = pxk::seg_radix_tile(dtype);
+  uint64_t span = 0;
+  for (uint32_t s = 0; s < sp.nseg; ++s) span = std::max<uint64_t>(span, sp.hi[s]);
+  const bool radix = pxk::tune().sort_radix_min > 0 && span < (1ull << 32) - (1ull << 20);
+  std::vector<uint64_t> blo(sp.nseg), bhi(sp.nseg), rlo, rhi;
+  std::vector<uint32_t> rfirst, rtseg;
+  for (uint32_t s = 0; s < sp.nseg; ++s) {
+    const uint64_t len = sp.hi[s] - sp.lo[s];
+    blo[s] = sp.lo[s];
+    bhi[s] = sp.hi[s];
+    if (radix && len > long_min) {
+      bhi[s] = sp.lo[s];  // not on the bitonic path
+      const uint32_t ls = static_cast<uint32_t>(rlo.size());
+      rlo.push_back(sp.lo[s]);
+      rhi.push_back(sp.hi[s]);
+      rfirst.push_back(static_cast<uint32_t>(rtseg.size()));
+      for (uint64_t k = 0; k < (len + rtile - 1) / rtile; ++k) rtseg.push_back(ls);
+    }
+  }
   pxk::SegSortPlan p;
-  if (!pxk::seg_sort_plan(dtype, sp.lo, sp.hi, sp.nseg, p)) return PX_ERR_ARG;
-  // blob: interleaved bounds (lo, hi) as u64, then the block tables (u32)
+  if (!pxk::seg_sort_plan(dtype, blo.data(), bhi.data(), sp.nseg, p)) return PX_ERR_ARG;
+  // blob: interleaved bitonic bounds (lo, hi) as u64, the bitonic block tables (u32),
+  // then the radix tables: seg_lo, seg_hi (u64), seg_first, tile_seg (u32)
+  const uint32_t nl = static_cast<uint32_t>(rlo.size());
   const size_t b_b = static_cast<size_t>(sp.nseg) * 2 * sizeof(uint64_t);
-  std::vector<unsigned char> blob(b_b + p.tables.size() * sizeof(uint32_t));
+  const size_t o_r = (b_b + p.tables.size() * sizeof(uint32_t) + 15) & ~size_t(15);
+  const size_t o_rhi = o_r + nl * sizeof(uint64_t);
+  const size_t o_rfirst = o_rhi + nl * sizeof(uint64_t);
+  const size_t o_rtseg = o_rfirst + nl * sizeof(uint32_t);
+  std::vector<unsigned char> blob(o_rtseg + rtseg.size() * sizeof(uint32_t));
   uint64_t* bounds = reinterpret_cast<uint64_t*>(blob.data());
   for (uint32_t s = 0; s < sp.nseg; ++s) {
-    bounds[2 * s] = sp.lo[s];
-    bounds[2 * s + 1] = sp.hi[s];
+    bounds[2 * s] = blo[s];
+    bounds[2 * s + 1] = bhi[s];
   }
   if (!p.tables.empty())
     std::memcpy(blob.data() + b_b, p.tables.data(), p.tables.size() * sizeof(uint32_t));
+  if (nl) {
+    std::memcpy(blob.data() + o_r, rlo.data(), nl * sizeof(uint64_t));
+    std::memcpy(blob.data() + o_rhi, rhi.data(), nl * sizeof(uint64_t));
+    std::memcpy(blob.data() + o_rfirst, rfirst.data(), nl * sizeof(uint32_t));
+    std::memcpy(blob.data() + o_rtseg, rtseg.data(), rtseg.size() * sizeof(uint32_t));
+  }
+  const size_t cnt_off = (span * pxk::dtype_size(dtype) + 255) & ~size_t(255);
+  if (nl && !c.reserve(static_cast<size_t>(rtseg.size()) * 256,
+                       cnt_off + pxk::seg_radix_cnt_bytes(dtype, nl)))
+    return PX_ERR_NOMEM;
   void* dev = nullptr;
   if (!c.upload(blob.data(), blob.size(), &dev)) return PX_ERR_NOMEM;
-  const uint64_t* bd = static_cast<const uint64_t*>(dev);
-  const uint32_t* tab_d = reinterpret_cast<const uint32_t*>(static_cast<unsigned char*>(dev) + b_b);
-  return dev_err(pxk::seg_sort_run(p, data, bd, tab_d, c.stream()), "seg_sort");
+  unsigned char* d = static_cast<unsigned char*>(dev);
+  const uint64_t* bd = reinterpret_cast<const uint64_t*>(d);
+  const uint32_t* tab_d = reinterpret_cast<const uint32_t*>(d + b_b);
+  int rc = dev_err(pxk::seg_sort_run(p, data, bd, tab_d, c.stream()), "seg_sort");
+  if (rc != PX_OK || nl == 0) return rc;
+  for (size_t i = 0; i < pxk::dtype_size(dtype); ++i)
+    if (!c.note_status_launch()) return PX_ERR_DEVICE;
+  const pxk::Scratch sc = c.scratch();
+  pxk::RadixSegs g;
+  g.ntiles = static_cast<uint32_t>(rtseg.size());
+  g.nseg = nl;
+  g.seg_lo = reinterpret_cast<const uint64_t*>(d + o_r);
+  g.seg_hi = reinterpret_cast<const uint64_t*>(d + o_rhi);
+  g.seg_first = reinterpret_cast<const uint32_t*>(d + o_rfirst);
+  g.tile_seg = reinterpret_cast<const uint32_t*>(d + o_rtseg);
+  g.seg_cnt = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(sc.partials) + cnt_off);
+  return dev_err(pxk::seg_radix_sort(dtype, data, span, g, sc, c.stream()), "seg_radix_sort");
 }
 
 }  // namespace parallax

@@ -183,10 +183,18 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 
 // ---------------------------------------------------------------- one digit pass
 // Four waves per SIMD (128 VGPRs): 1024 threads per CU whatever the block size.
-template <class T, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_radix_pass(const T* __restrict__ in,
-                                                    T* __restrict__ out, uint64_t n,
-                                                    int shift, uint64_t* status, Ctl* ctl) {
+//
+// SEG = false: one array [0, n); reads `a`, writes `b`; bin starts from the control
+// block.  SEG = true: the tiles of the long segments of a batch (RadixSegs tables; a
+// tile never spans two segments); bin starts from the segment's own digit counts; a
+// tile looks back only over its segment's tiles; a digit that every key shares is
+// skipped (no block does anything), and the control block's selector says which of
+// `a` (the array) and `b` (the scratch copy) holds the keys: the last block of a pass
+// flips it.
+template <class T, int BLOCK, bool SEG>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_radix_pass(
+    T* __restrict__ a, T* __restrict__ b, uint64_t n, int shift, RadixSegs g,
+    uint64_t* status, Ctl* ctl) {
   static_assert(BLOCK >= kBins, "one thread per digit in the count phases");
   static_assert(BLOCK * kKpt<T> <= 65536, "ranks are kept in 16 bits");
   constexpr int KPT = kKpt<T>;
@@ -194,13 +202,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   constexpr int WSPAN = KPT * kWave;
   constexpr int TILE = BLOCK * KPT;
   constexpr int R = kLookWindow;
-  const uint32_t ntiles = static_cast<uint32_t>((n + TILE - 1) / TILE);
+  constexpr int D = sizeof(T);
+  const uint32_t ntiles = SEG ? g.ntiles : static_cast<uint32_t>((n + TILE - 1) / TILE);
+  if constexpr (SEG) {
+    if ((ld_agent32(&ctl->radix_skip) >> (shift / 8)) & 1u) return;  // every key shares this digit
+  }
+  const bool swapped = SEG && ld_agent32(&ctl->radix_sel) != 0u;
+  const T* __restrict__ in = swapped ? b : a;
+  T* __restrict__ out = swapped ? a : b;
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
   __shared__ uint32_t s_wh[NW][kBins];  // wave digit counts, then tile-local starts
   __shared__ uint32_t s_goff[kBins];    // output index of tile-local index 0, per digit
   __shared__ uint32_t s_wsum[kBins / kWave];
+  __shared__ uint32_t s_bsum[kBins / kWave];  // segment bin-start scan (SEG)
   __shared__ T s_keys[TILE];
 
   const int lane = lane_id();
@@ -221,15 +237,30 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t tile = s_tile;
     const uint32_t epoch = s_epoch;
     if (tile >= ntiles) break;
-    // Positions in 32-bit arithmetic (n < 2^32): key j of this lane is real when j < nv.
+    // Positions in 32-bit arithmetic (n < 2^32; hipcc lowered a 64-bit form of the
+    // tile-length select -- a vector compare, then a scalar select on SCC -- to a select
+    // on the subtraction's borrow, giving a partial last tile its full length): key j of
+    // this lane is real when j < nv.
     // Positions past n hold the pad key: digit 255 in every pass and last in input
     // order, so they take the tile's last local indices [tn, TILE), which are never
     // written out (only the last tile has them, and nothing reads its counts).
-    const uint32_t n32 = static_cast<uint32_t>(n);
-    const uint32_t tbase = tile * static_cast<uint32_t>(TILE);
+    // The tile covers positions [tbase, tend) of the array.
+    uint32_t tbase, tend, first_tile = 0, seg = 0, seg_lo = 0;
+    if constexpr (SEG) {
+      seg = g.tile_seg[tile];
+      first_tile = g.seg_first[seg];
+      seg_lo = static_cast<uint32_t>(g.seg_lo[seg]);
+      const uint32_t seg_hi = static_cast<uint32_t>(g.seg_hi[seg]);
+      tbase = seg_lo + (tile - first_tile) * static_cast<uint32_t>(TILE);
+      tend = seg_hi - tbase < static_cast<uint32_t>(TILE) ? seg_hi : tbase + static_cast<uint32_t>(TILE);
+    } else {
+      const uint32_t n32 = static_cast<uint32_t>(n);
+      tbase = tile * static_cast<uint32_t>(TILE);
+      tend = n32 - tbase < static_cast<uint32_t>(TILE) ? n32 : tbase + static_cast<uint32_t>(TILE);
+    }
     const uint32_t wbase = tbase + static_cast<uint32_t>(wid * WSPAN);
     const uint32_t first = wbase + static_cast<uint32_t>(lane);
-    const uint32_t left = first < n32 ? (n32 - first + kWave - 1) / kWave : 0u;
+    const uint32_t left = first < tend ? (tend - first + kWave - 1) / kWave : 0u;
     const int nv = left < static_cast<uint32_t>(KPT) ? static_cast<int>(left) : KPT;
 
     // ---- loads (a wave's key j of all lanes is 64 consecutive keys)
@@ -281,9 +312,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         tot += c;
       }
       st_agent(&status[static_cast<uint64_t>(tile) * kBins + b],
-               status_word(epoch, tile == 0 ? kStIncl : kStAgg, tot));
+               status_word(epoch, tile == first_tile ? kStIncl : kStAgg, tot));
       incl = wave_incl_scan(tot);
       if (lane == kWave - 1) s_wsum[wid] = incl;
+    }
+    uint32_t scount = 0, sincl = 0;  // the segment's count of this digit (SEG)
+    if constexpr (SEG) {
+      if (threadIdx.x < kBins) {
+        scount = g.seg_cnt[(static_cast<uint64_t>(seg) * D + shift / 8) * kBins + threadIdx.x];
+        sincl = wave_incl_scan(scount);
+        if (lane == kWave - 1) s_bsum[wid] = sincl;
+      }
     }
     __syncthreads();
 
@@ -309,17 +348,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (threadIdx.x < kBins) {
       const int b = threadIdx.x;
       uint32_t prefix = 0;
-      if (tile > 0) {
+      if (tile > first_tile) {
+        const int64_t tstop = first_tile;  // the segment's (or array's) first tile
         int64_t t = static_cast<int64_t>(tile) - 1;
         while (true) {
           uint64_t wv[R];
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            wv[r] = (t - r >= 0) ? ld_agent(&status[static_cast<uint64_t>(t - r) * kBins + b]) : 0ull;
+            wv[r] = (t - r >= tstop) ? ld_agent(&status[static_cast<uint64_t>(t - r) * kBins + b]) : 0ull;
           bool done = false;
           int r = 0;
           for (; r < R; ++r) {
-            if (t - r < 0) {
+            if (t - r < tstop) {
               done = true;
               break;
             }
@@ -338,16 +378,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         st_agent(&status[static_cast<uint64_t>(tile) * kBins + b],
                  status_word(epoch, kStIncl, prefix + tot));
       }
-      s_goff[b] = bins[b] + prefix - lstart;  // modulo 2^32; n < 2^32
+      uint32_t bin;
+      if constexpr (SEG) {
+        bin = seg_lo + sincl - scount;
+        for (int w = 0; w < wid; ++w) bin += s_bsum[w];
+      } else {
+        bin = bins[b];
+      }
+      s_goff[b] = bin + prefix - lstart;  // modulo 2^32; n < 2^32
     }
     __syncthreads();
 
     // ---- write out: consecutive tile-local indices of one digit are consecutive outputs
-    // Keys of this tile, in 32-bit arithmetic (n < 2^32).  hipcc lowered the 64-bit form
-    // of this select (a vector compare, then a scalar select on SCC) to a select on the
-    // borrow of the subtraction, which gave the full tile length for a partial last tile.
-    const uint32_t tleft = n32 - tbase;
-    const uint32_t tn = tleft < static_cast<uint32_t>(TILE) ? tleft : static_cast<uint32_t>(TILE);
+    const uint32_t tn = tend - tbase;
     if (tn == TILE) {
 #pragma unroll
       for (int i0 = 0; i0 < KPT; ++i0) {
@@ -363,7 +406,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) block_end(ctl, s_epoch);
+  if (threadIdx.x == 0) {
+    const uint32_t d = __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (d == gridDim.x - 1) {  // last block: reset for the next launch, flip the selector
+      if (SEG) st_agent32(&ctl->radix_sel, ld_agent32(&ctl->radix_sel) ^ 1u);
+      st_agent32(&ctl->ticket, 0u);
+      st_agent32(&ctl->done, 0u);
+      st_agent32(&ctl->epoch, next_epoch(s_epoch));
+    }
+  }
 }
 
 // Threads per digit-pass block (256, 512 or 1024); blocks per CU = 1024 / block.
@@ -381,7 +432,7 @@ hipError_t radix_passes_t(T* data, T* tmp, uint64_t n, uint32_t grid, const Scra
   T* b = tmp;
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
     hook("sort_radix", 0);
-    k_radix_pass<T, BLOCK><<<grid, BLOCK, 0, stream>>>(a, b, n, 8 * d, s.status, s.ctl);
+    k_radix_pass<T, BLOCK, false><<<grid, BLOCK, 0, stream>>>(a, b, n, 8 * d, RadixSegs{}, s.status, s.ctl);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hook("sort_radix", 1);
@@ -422,6 +473,181 @@ hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t 
   return radix_passes_t<T, 512>(data, tmp, n, grid, s, stream, hook);
 }
 
+// ---------------------------------------------------------------- segmented
+constexpr int kSegBlock = 1024;  // digit-pass block of the segmented sort
+constexpr int kSegUnroll = 8;    // loads in flight per thread in the prologue kernels
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * 256)
+    p[i] = 0u;
+}
+
+// Tile t of the long segments covers [tbase, tend).
+template <class T>
+__device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
+                                         uint32_t& tbase, uint32_t& tend) {
+  constexpr uint32_t TILE = kSegBlock * kKpt<T>;
+  seg = g.tile_seg[tile];
+  const uint32_t lo = static_cast<uint32_t>(g.seg_lo[seg]);
+  const uint32_t hi = static_cast<uint32_t>(g.seg_hi[seg]);
+  tbase = lo + (tile - g.seg_first[seg]) * TILE;
+  tend = hi - tbase < TILE ? hi : tbase + TILE;
+}
+
+// OR of the encoded keys and of their complements over the long segments.  The last
+// block marks the digits every key shares (their passes are skipped), sets the
+// selector to the parity of the passes that will run (odd: the histogram kernel copies
+// the keys to the scratch copy, so that the last pass writes the array) and clears the
+// ORs for the next call.
+template <class T>
+__global__ __launch_bounds__(256) void k_segradix_or(const T* data, RadixSegs g, Ctl* ctl) {
+  constexpr int D = sizeof(T);
+  using U = typename RadixKey<T>::U;
+  __shared__ unsigned long long s_or, s_nor;
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    s_or = 0ull;
+    s_nor = 0ull;
+  }
+  __syncthreads();
+  U acc_or = 0, acc_nor = 0;
+  for (uint32_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
+    uint32_t seg, tbase, tend;
+    seg_tile<T>(g, tile, seg, tbase, tend);
+    for (uint32_t p0 = tbase + threadIdx.x; p0 < tend; p0 += 256 * kSegUnroll) {
+      T x[kSegUnroll];  // independent loads in flight
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; ++u) {
+        const uint32_t p = p0 + u * 256;
+        x[u] = p < tend ? data[p] : data[p0];
+      }
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; ++u) {
+        const U k = RadixKey<T>::enc(x[u]);
+        acc_or |= k;
+        acc_nor |= static_cast<U>(~k);
+      }
+    }
+  }
+  atomicOr(&s_or, static_cast<unsigned long long>(acc_or));
+  atomicOr(&s_nor, static_cast<unsigned long long>(acc_nor));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_or(&ctl->radix_or, static_cast<uint64_t>(s_or), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(&ctl->radix_nor, static_cast<uint64_t>(s_nor), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const uint32_t d = __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (d == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  __threadfence();
+  const uint64_t varying = ld_agent(&ctl->radix_or) & ld_agent(&ctl->radix_nor);  // set in some key, clear in another
+  uint32_t skip = 0, runs = 0;
+  for (int d = 0; d < D; ++d) {
+    if (((varying >> (8 * d)) & 0xffu) == 0u) skip |= 1u << d;
+    else ++runs;
+  }
+  st_agent32(&ctl->radix_skip, skip);
+  st_agent32(&ctl->radix_sel, runs & 1u);
+  st_agent(&ctl->radix_or, 0ull);
+  st_agent(&ctl->radix_nor, 0ull);
+  st_agent32(&ctl->done, 0u);
+}
+
+// Counts of the digits that run, per long segment.  Each block takes a contiguous range
+// of tiles and keeps counting in LDS (a copy per wave) while consecutive tiles belong to
+// the same segment; it adds the counts to the segment's totals when the segment changes
+// and at the end (agent-scope adds, since a segment's tiles may span blocks).  When an
+// odd number of digits runs the keys are also copied to the scratch copy here.
+template <class T>
+__global__ __launch_bounds__(256) void k_segradix_hist(const T* data, T* tmp, RadixSegs g, Ctl* ctl) {
+  constexpr int D = sizeof(T);
+  constexpr int HB = 256;
+  constexpr int NWH = HB / kWave;
+  using U = typename RadixKey<T>::U;
+  __shared__ uint32_t h[NWH][D * kBins];
+
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  uint32_t* hw = h[wid];
+  const uint32_t skip = ld_agent32(&ctl->radix_skip);
+  const bool copy = ld_agent32(&ctl->radix_sel) != 0u;
+  const uint32_t t0 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * g.ntiles / gridDim.x);
+  const uint32_t t1 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x + 1) * g.ntiles / gridDim.x);
+  auto flush = [&](uint32_t seg) {
+    __syncthreads();
+    uint32_t* cnt = g.seg_cnt + static_cast<uint64_t>(seg) * D * kBins;
+    for (int i = threadIdx.x; i < D * kBins; i += HB) {
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < NWH; ++w) c += h[w][i];
+      if (c) __hip_atomic_fetch_add(&cnt[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;
+  };
+  for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;  // the wave's own copy
+  uint32_t cur = 0xffffffffu;
+  for (uint32_t tile = t0; tile < t1; ++tile) {
+    uint32_t seg, tbase, tend;
+    seg_tile<T>(g, tile, seg, tbase, tend);
+    if (seg != cur) {
+      if (cur != 0xffffffffu) flush(cur);
+      cur = seg;
+    }
+    for (uint32_t p0 = tbase + threadIdx.x; p0 < tend; p0 += HB * kSegUnroll) {
+      T x[kSegUnroll];  // independent loads in flight
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; ++u) {
+        const uint32_t p = p0 + u * HB;
+        if (p < tend) x[u] = data[p];
+      }
+#pragma unroll
+      for (int u = 0; u < kSegUnroll; ++u) {
+        const uint32_t p = p0 + u * HB;
+        if (p < tend) {
+          if (copy) tmp[p] = x[u];
+          const U k = RadixKey<T>::enc(x[u]);
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+            if (!((skip >> d) & 1u)) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
+        }
+      }
+    }
+  }
+  if (cur != 0xffffffffu) flush(cur);
+}
+
+template <class T>
+hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, const Scratch& s,
+                            hipStream_t stream) {
+  if (g.ntiles == 0) return hipSuccess;
+  T* data = static_cast<T*>(data_v);
+  T* tmp = static_cast<T*>(s.partials);
+  if (span >= (1ull << 32) || static_cast<uint64_t>(g.ntiles) * kBins > s.status_words ||
+      s.partial_bytes < span * sizeof(T) || !tmp || !g.seg_cnt)
+    return hipErrorInvalidValue;
+  const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
+  const uint64_t ncnt = static_cast<uint64_t>(g.nseg) * sizeof(T) * kBins;
+  k_fill_u32<<<static_cast<uint32_t>(std::min<uint64_t>((ncnt + 255) / 256, 1024)), 256, 0, stream>>>(g.seg_cnt, ncnt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint32_t ogrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 8ull * cus));
+  k_segradix_or<T><<<ogrid, 256, 0, stream>>>(data, g, s.ctl);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 4ull * cus));
+  k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, tmp, g, s.ctl);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, cus));
+  for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
+    k_radix_pass<T, kSegBlock, true><<<grid, kSegBlock, 0, stream>>>(data, tmp, span, 8 * d, g, s.status, s.ctl);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 }  // namespace
 
 uint64_t radix_min() {
@@ -439,6 +665,25 @@ uint64_t radix_status_words(int dtype, uint64_t n) {
 }
 
 int radix_passes(int dtype) { return static_cast<int>(dtype_size(dtype)); }
+
+uint64_t seg_radix_min_len(int dtype) {
+  // one LDS tile of the bitonic path: shorter segments sort there in one launch
+  return dtype_size(dtype) == 4 ? (1ull << 14) : (1ull << 13);
+}
+
+uint32_t seg_radix_tile(int dtype) {
+  return static_cast<uint32_t>(kSegBlock * (dtype_size(dtype) == 4 ? kKpt<uint32_t> : kKpt<uint64_t>));
+}
+
+uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg) {
+  return static_cast<uint64_t>(nseg) * dtype_size(dtype) * kBins * sizeof(uint32_t);
+}
+
+hipError_t seg_radix_sort(int dtype, void* data, uint64_t span, const RadixSegs& seg,
+                          const Scratch& s, hipStream_t stream) {
+  PXK_DISPATCH(dtype, return seg_radix_sort_t<T>(data, span, seg, s, stream));
+  return hipErrorInvalidValue;
+}
 
 hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipStream_t stream,
                       const LaunchHook& hook) {

@@ -87,6 +87,68 @@ def test_seg_sort(gpu_ctx, dt):
     assert d.download(a.dtype, a.size).tobytes() == pxo.seg_sort(a, off).tobytes()
 
 
+
+def radix_seg_input(dt, seed, kind):
+    """Short and long segments mixed (long ones take the segmented radix path)."""
+    rng = np.random.default_rng(seed)
+    lens = np.array([5, 20_000, 0, 8_192, 8_193, 16_384, 16_385, 3, 150_001, 40_000, 1, 70_000, 9_000, 33_000],
+                    dtype=np.int64)
+    off = datagen.offsets_from_lengths(lens)
+    n = int(off[-1])
+    t = NP_DTYPES[dt]
+    if kind == "full":
+        if dt in FLOATS:
+            a = (rng.standard_normal(n) * np.exp(rng.uniform(-20, 20, n))).astype(t)
+        else:
+            info = np.iinfo(t)
+            a = rng.integers(info.min, info.max, n, dtype=t, endpoint=True)
+    elif kind == "narrow":  # 20 significant bits: three digits run, a copy back follows
+        a = rng.integers(0, 1 << 20, n).astype(t) if dt not in FLOATS else rng.integers(0, 1 << 20, n).astype(t)
+    else:  # "equal": long segments of one value each (every digit skipped) + short random ones
+        a = np.repeat(rng.integers(0, 1000, lens.size).astype(t), lens)
+    return a, off
+
+
+@pytest.mark.parametrize("kind", ["full", "narrow", "equal"])
+@pytest.mark.parametrize("dt", ALL)
+def test_seg_sort_radix(gpu_ctx, dt, kind):
+    a, off = radix_seg_input(dt, 60 + dt, kind)
+    d = DeviceBuffer.from_array(a)
+    check(lib.pxb_seg_sort(gpu_ctx.h, dt, d.ptr, u64p(off), off.size - 1), "seg_sort")
+    gpu_ctx.sync()
+    assert d.download(a.dtype, a.size).tobytes() == pxo.seg_sort(a, off).tobytes()
+    # again on the same context (scratch, counters and selector reused)
+    b = a[::-1].copy()
+    d2 = DeviceBuffer.from_array(b)
+    check(lib.pxb_seg_sort(gpu_ctx.h, dt, d2.ptr, u64p(off), off.size - 1), "seg_sort")
+    gpu_ctx.sync()
+    assert d2.download(b.dtype, b.size).tobytes() == pxo.seg_sort(b, off).tobytes()
+
+
+@pytest.mark.parametrize("dt", [PX_U64, PX_F32])
+def test_seg_sort_radix_plan_replays(gpu_ctx, dt):
+    """A captured plan with long segments: every replay sorts fresh data, with eager
+    work on the same context between replays."""
+    a, off = radix_seg_input(dt, 70 + dt, "narrow" if dt == PX_U64 else "full")
+    d = DeviceBuffer.from_array(a)
+    plan = lib.pxb_plan_create(gpu_ctx.h, C.OP_SORT, dt, None, d.ptr, d.ptr, u64p(off), off.size - 1, None)
+    assert plan
+    try:
+        rng = np.random.default_rng(5)
+        for r in range(3):
+            x = rng.permutation(a)
+            d.upload(x)
+            check(lib.pxb_plan_launch(plan), "replay")
+            gpu_ctx.sync()
+            assert d.download(a.dtype, a.size).tobytes() == pxo.seg_sort(x, off).tobytes(), r
+            y = rng.permutation(a)
+            e = DeviceBuffer.from_array(y)
+            check(lib.pxb_seg_sort(gpu_ctx.h, dt, e.ptr, u64p(off), off.size - 1), "eager")
+            gpu_ctx.sync()
+            assert e.download(a.dtype, a.size).tobytes() == pxo.seg_sort(y, off).tobytes(), r
+    finally:
+        lib.pxb_plan_destroy(plan)
+
 @pytest.mark.parametrize("dt", ALL)
 def test_seg_copy_if_and_unique(gpu_ctx, dt):
     a, off = seg_input(dt, 40 + dt)
