@@ -511,7 +511,8 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
     std::memcpy(blob.data() + o_rfirst, rfirst.data(), nl * sizeof(uint32_t));
     std::memcpy(blob.data() + o_rtseg, rtseg.data(), rtseg.size() * sizeof(uint32_t));
   }
-  const size_t cnt_off = (span * pxk::dtype_size(dtype) + 255) & ~size_t(255);
+  const size_t buf_bytes = (span * pxk::dtype_size(dtype) + 255) & ~size_t(255);
+  const size_t cnt_off = 2 * buf_bytes;  // scratch copy, third buffer, then the counts
   if (nl && !c.reserve(static_cast<size_t>(rtseg.size()) * 256,
                        cnt_off + pxk::seg_radix_cnt_bytes(dtype, nl)))
     return PX_ERR_NOMEM;
@@ -533,6 +534,7 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
   g.seg_first = reinterpret_cast<const uint32_t*>(d + o_rfirst);
   g.tile_seg = reinterpret_cast<const uint32_t*>(d + o_rtseg);
   g.seg_cnt = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(sc.partials) + cnt_off);
+  g.tmp2 = static_cast<unsigned char*>(sc.partials) + buf_bytes;
   return dev_err(pxk::seg_radix_sort(dtype, data, span, g, sc, c.stream()), "seg_radix_sort");
 }
 

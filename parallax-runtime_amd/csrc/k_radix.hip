@@ -188,9 +188,10 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 // block.  SEG = true: the tiles of the long segments of a batch (RadixSegs tables; a
 // tile never spans two segments); bin starts from the segment's own digit counts; a
 // tile looks back only over its segment's tiles; a digit that every key shares is
-// skipped (no block does anything), and the control block's selector says which of
-// `a` (the array) and `b` (the scratch copy) holds the keys: the last block of a pass
-// flips it.
+// skipped (no block does anything).  The k digits that run move the keys between `a`
+// (the array), `b` and g.tmp2 so that the last one writes the array and none reads and
+// writes the same buffer: k even a -> b -> a ..; k odd (>= 3) a -> b -> tmp2 -> b ..
+// -> a; k = 1 b -> a (the histogram kernel copied the keys to b).
 template <class T, int BLOCK, bool SEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_radix_pass(
     T* __restrict__ a, T* __restrict__ b, uint64_t n, int shift, RadixSegs g,
@@ -204,12 +205,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   constexpr int R = kLookWindow;
   constexpr int D = sizeof(T);
   const uint32_t ntiles = SEG ? g.ntiles : static_cast<uint32_t>((n + TILE - 1) / TILE);
+  const T* __restrict__ in = a;
+  T* __restrict__ out = b;
   if constexpr (SEG) {
-    if ((ld_agent32(&ctl->radix_skip) >> (shift / 8)) & 1u) return;  // every key shares this digit
+    const uint32_t skip = ld_agent32(&ctl->radix_skip);
+    const int dig = shift / 8;
+    if ((skip >> dig) & 1u) return;  // every key shares this digit
+    const uint32_t runs = ~skip & ((1u << D) - 1u);
+    const int k = __builtin_popcount(runs);
+    const int i = __builtin_popcount(runs & ((1u << dig) - 1u));  // this pass's run index
+    T* const buf[3] = {a, b, static_cast<T*>(g.tmp2)};
+    auto dst = [&](int j) { return j == k - 1 ? 0 : ((k & 1) ? 1 + (j & 1) : 1 - (j & 1)); };
+    in = buf[i == 0 ? (k == 1 ? 1 : 0) : dst(i - 1)];
+    out = buf[dst(i)];
   }
-  const bool swapped = SEG && ld_agent32(&ctl->radix_sel) != 0u;
-  const T* __restrict__ in = swapped ? b : a;
-  T* __restrict__ out = swapped ? a : b;
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
@@ -408,8 +417,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   }
   if (threadIdx.x == 0) {
     const uint32_t d = __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (d == gridDim.x - 1) {  // last block: reset for the next launch, flip the selector
-      if (SEG) st_agent32(&ctl->radix_sel, ld_agent32(&ctl->radix_sel) ^ 1u);
+    if (d == gridDim.x - 1) {  // last block: reset for the next launch
       st_agent32(&ctl->ticket, 0u);
       st_agent32(&ctl->done, 0u);
       st_agent32(&ctl->epoch, next_epoch(s_epoch));
@@ -495,11 +503,11 @@ __device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint
   tend = hi - tbase < TILE ? hi : tbase + TILE;
 }
 
-// OR of the encoded keys and of their complements over the long segments.  The last
-// block marks the digits every key shares (their passes are skipped), sets the
-// selector to the parity of the passes that will run (odd: the histogram kernel copies
-// the keys to the scratch copy, so that the last pass writes the array) and clears the
-// ORs for the next call.
+// OR of the encoded keys and of their complements over the long segments (16-byte
+// loads; a tile's unaligned head and tail one key at a time).  The last block marks the
+// digits every key shares (their passes are skipped), sets the selector when exactly
+// one digit runs (the histogram kernel then copies the keys to the scratch copy, which
+// that pass reads) and clears the ORs for the next call.
 template <class T>
 __global__ __launch_bounds__(256) void k_segradix_or(const T* data, RadixSegs g, Ctl* ctl) {
   constexpr int D = sizeof(T);
@@ -511,31 +519,47 @@ __global__ __launch_bounds__(256) void k_segradix_or(const T* data, RadixSegs g,
     s_nor = 0ull;
   }
   __syncthreads();
+  constexpr int VEC = 16 / sizeof(T);
+  using V = Vec<T, VEC>;
   U acc_or = 0, acc_nor = 0;
+  auto add = [&](T x) {
+    const U k = RadixKey<T>::enc(x);
+    acc_or |= k;
+    acc_nor |= static_cast<U>(~k);
+  };
   for (uint32_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     uint32_t seg, tbase, tend;
     seg_tile<T>(g, tile, seg, tbase, tend);
-    for (uint32_t p0 = tbase + threadIdx.x; p0 < tend; p0 += 256 * kSegUnroll) {
-      T x[kSegUnroll];  // independent loads in flight
+    const uint32_t mis = static_cast<uint32_t>(((reinterpret_cast<uintptr_t>(data + tbase)) & 15u) / sizeof(T));
+    uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
+    if (head > tend - tbase) head = tend - tbase;
+    const uint32_t nvec = (tend - tbase - head) / VEC;
+    const uint32_t body_end = tbase + head + nvec * VEC;
+    if (threadIdx.x < head) add(data[tbase + threadIdx.x]);
+    if (threadIdx.x < tend - body_end) add(data[body_end + threadIdx.x]);
+    const V* vp = reinterpret_cast<const V*>(data + tbase + head);
+    for (uint32_t v0 = threadIdx.x; v0 < nvec; v0 += 256 * kSegUnroll) {
+      V x[kSegUnroll];  // independent loads in flight
 #pragma unroll
       for (int u = 0; u < kSegUnroll; ++u) {
-        const uint32_t p = p0 + u * 256;
-        x[u] = p < tend ? data[p] : data[p0];
+        const uint32_t v = v0 + u * 256;
+        x[u] = ld_stream(vp + (v < nvec ? v : v0));
       }
 #pragma unroll
-      for (int u = 0; u < kSegUnroll; ++u) {
-        const U k = RadixKey<T>::enc(x[u]);
-        acc_or |= k;
-        acc_nor |= static_cast<U>(~k);
-      }
+      for (int u = 0; u < kSegUnroll; ++u)
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) add(x[u].v[j]);
     }
   }
   atomicOr(&s_or, static_cast<unsigned long long>(acc_or));
   atomicOr(&s_nor, static_cast<unsigned long long>(acc_nor));
   __syncthreads();
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_or(&ctl->radix_or, static_cast<uint64_t>(s_or), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_or(&ctl->radix_nor, static_cast<uint64_t>(s_nor), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // 16 slots: the blocks ending together do not all update one word
+    __hip_atomic_fetch_or(&ctl->radix_or[blockIdx.x & 15u], static_cast<uint64_t>(s_or), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_or(&ctl->radix_nor[blockIdx.x & 15u], static_cast<uint64_t>(s_nor), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
     const uint32_t d = __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (d == gridDim.x - 1) ? 1 : 0;
@@ -543,24 +567,29 @@ __global__ __launch_bounds__(256) void k_segradix_or(const T* data, RadixSegs g,
   __syncthreads();
   if (!s_last || threadIdx.x != 0) return;
   __threadfence();
-  const uint64_t varying = ld_agent(&ctl->radix_or) & ld_agent(&ctl->radix_nor);  // set in some key, clear in another
+  uint64_t all_or = 0, all_nor = 0;
+  for (int q = 0; q < 16; ++q) {
+    all_or |= ld_agent(&ctl->radix_or[q]);
+    all_nor |= ld_agent(&ctl->radix_nor[q]);
+    st_agent(&ctl->radix_or[q], 0ull);
+    st_agent(&ctl->radix_nor[q], 0ull);
+  }
+  const uint64_t varying = all_or & all_nor;  // set in some key, clear in another
   uint32_t skip = 0, runs = 0;
   for (int d = 0; d < D; ++d) {
     if (((varying >> (8 * d)) & 0xffu) == 0u) skip |= 1u << d;
     else ++runs;
   }
   st_agent32(&ctl->radix_skip, skip);
-  st_agent32(&ctl->radix_sel, runs & 1u);
-  st_agent(&ctl->radix_or, 0ull);
-  st_agent(&ctl->radix_nor, 0ull);
+  st_agent32(&ctl->radix_sel, runs == 1u ? 1u : 0u);
   st_agent32(&ctl->done, 0u);
 }
 
 // Counts of the digits that run, per long segment.  Each block takes a contiguous range
 // of tiles and keeps counting in LDS (a copy per wave) while consecutive tiles belong to
 // the same segment; it adds the counts to the segment's totals when the segment changes
-// and at the end (agent-scope adds, since a segment's tiles may span blocks).  When an
-// odd number of digits runs the keys are also copied to the scratch copy here.
+// and at the end (agent-scope adds, since a segment's tiles may span blocks).  When
+// exactly one digit runs the keys are also copied to the scratch copy here.
 template <class T>
 __global__ __launch_bounds__(256) void k_segradix_hist(const T* data, T* tmp, RadixSegs g, Ctl* ctl) {
   constexpr int D = sizeof(T);
@@ -627,14 +656,14 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
   T* data = static_cast<T*>(data_v);
   T* tmp = static_cast<T*>(s.partials);
   if (span >= (1ull << 32) || static_cast<uint64_t>(g.ntiles) * kBins > s.status_words ||
-      s.partial_bytes < span * sizeof(T) || !tmp || !g.seg_cnt)
+      s.partial_bytes < 2 * span * sizeof(T) || !tmp || !g.tmp2 || !g.seg_cnt)
     return hipErrorInvalidValue;
   const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
   const uint64_t ncnt = static_cast<uint64_t>(g.nseg) * sizeof(T) * kBins;
   k_fill_u32<<<static_cast<uint32_t>(std::min<uint64_t>((ncnt + 255) / 256, 1024)), 256, 0, stream>>>(g.seg_cnt, ncnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint32_t ogrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 8ull * cus));
+  const uint32_t ogrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 2ull * cus));
   k_segradix_or<T><<<ogrid, 256, 0, stream>>>(data, g, s.ctl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 4ull * cus));

@@ -26,10 +26,10 @@ struct Ctl {
   uint64_t result;  // kept count written by the last compaction tile
   uint64_t pad1[5];
   uint8_t value[64];  // reduce result slot (one element, written by the last block)
-  uint32_t radix_sel;   // segmented radix: 1 when the keys are in the scratch copy
+  uint32_t radix_sel;   // segmented radix: 1 when one digit runs (keys copied to scratch)
   uint32_t radix_skip;  // segmented radix: bit d set when every key shares digit d
-  uint64_t radix_or;    // segmented radix: OR of the encoded keys (zero between calls)
-  uint64_t radix_nor;   // ... OR of their complements (zero between calls)
+  uint64_t radix_or[16];   // segmented radix: OR of the encoded keys, 16 slots (block % 16)
+  uint64_t radix_nor[16];  // ... OR of their complements; all zero between calls
   uint32_t radix_hist[8 * 256];  // radix sort digit counts (zero between calls)
   uint32_t radix_bins[8 * 256];  // radix sort exclusive bin starts per digit position
 };
@@ -172,17 +172,19 @@ struct RadixSegs {
   const uint64_t* seg_lo = nullptr;     // long segment -> first element
   const uint64_t* seg_hi = nullptr;     // long segment -> end element
   uint32_t* seg_cnt = nullptr;          // long segment x digit position x 256 counts
+  void* tmp2 = nullptr;                 // third buffer (same indexing as the array)
 };
 bool radix_sort_ok(int dtype, uint64_t n);
 uint64_t radix_status_words(int dtype, uint64_t n);
 int radix_passes(int dtype);
 hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipStream_t stream,
                       const LaunchHook& hook = {});
-// Segmented radix sort of the long segments (kernels "seg_radix_hist", "seg_radix" per
-// digit -- digits every key shares are skipped on the device -- and a copy back when
-// an odd number of digits ran).  Scratch: seg_radix_status_words() status words;
-// partials hold the second copy (indices up to the largest segment end) and
-// seg.seg_cnt (set by the caller inside partials, seg_radix_cnt_bytes()).
+// Segmented radix sort of the long segments: an OR pass (digits every key shares are
+// skipped on the device), per-segment digit counts, then one pass per digit that runs,
+// the last writing the array.  Scratch: 256 status words per tile; partials hold
+// the second and third copies (indices up to the largest segment end; the third is
+// seg.tmp2) and seg.seg_cnt (both set by the caller inside partials,
+// seg_radix_cnt_bytes()).
 uint64_t seg_radix_min_len(int dtype);  // shorter segments stay on the bitonic path
 uint32_t seg_radix_tile(int dtype);
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg);
