@@ -125,6 +125,19 @@ def test_seg_sort_radix(gpu_ctx, dt, kind):
     assert d2.download(b.dtype, b.size).tobytes() == pxo.seg_sort(b, off).tobytes()
 
 
+@pytest.mark.parametrize("dt", ALL)
+def test_seg_sort_radix_unaligned(gpu_ctx, dt):
+    """Segmented radix with the array one element past a 16-byte boundary (the OR pass
+    reads each tile's unaligned head and tail one key at a time)."""
+    a, off = radix_seg_input(dt, 80 + dt, "full")
+    item = a.itemsize
+    d = DeviceBuffer(a.nbytes + 64)
+    d.upload(a, item)
+    check(lib.pxb_seg_sort(gpu_ctx.h, dt, d.addr(item), u64p(off), off.size - 1), "seg_sort")
+    gpu_ctx.sync()
+    assert d.download(a.dtype, a.size, item).tobytes() == pxo.seg_sort(a, off).tobytes()
+
+
 @pytest.mark.parametrize("dt", [PX_U64, PX_F32])
 def test_seg_sort_radix_plan_replays(gpu_ctx, dt):
     """A captured plan with long segments: every replay sorts fresh data, with eager
