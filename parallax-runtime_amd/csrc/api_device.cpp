@@ -161,7 +161,7 @@ void* pxd_ctx_stream(pxd_ctx_t ctx) { return ok_ctx(ctx) ? ctx->c->stream() : nu
 
 int pxd_sync(pxd_ctx_t ctx) {
   if (!ok_ctx(ctx)) return PX_EARG;
-  return hip_rc(hipStreamSynchronize(ctx->c->stream()), "stream sync");
+  return hip_rc(parallax::wait_stream(ctx->c->stream()), "stream sync");
 }
 
 int pxd_timer_begin(pxd_ctx_t ctx) {

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -68,6 +69,16 @@ StreamContext::~StreamContext() {
   if (pin_) (void)hipHostFree(pin_);
   if (pin_ev_) (void)hipEventDestroy(pin_ev_);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+hipError_t wait_stream(hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  while (true) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e != hipErrorNotReady) return e;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) break;
+  }
+  return hipStreamSynchronize(s);
 }
 
 bool StreamContext::grow(void** p, size_t* cap, size_t need, bool zero) {

@@ -16,6 +16,10 @@
 
 namespace parallax {
 
+// Waits for the stream's work: polls for up to ~200 us (a short call completes and is
+// seen within a microsecond or two), then blocks in the runtime for long ones.
+hipError_t wait_stream(hipStream_t s);
+
 // Error codes of the pxd_* / pxb_* / pxs_* entry points (0 = success).
 enum PxStatus {
   PX_OK = 0,

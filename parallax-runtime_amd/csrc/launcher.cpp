@@ -231,7 +231,7 @@ bool KernelLauncher::resolve(void* p, size_t bytes, bool read, Operand& op) {
 }
 
 bool KernelLauncher::finish(Operand& op, bool write) {
-  if (hipStreamSynchronize(ctx_->stream()) != hipSuccess) return false;
+  if (wait_stream(ctx_->stream()) != hipSuccess) return false;
   if (!write || op.bytes == 0) return true;
   if (op.kind == Operand::ARENA) return arena_->invalidate_range(op.user, op.bytes);
   if (op.kind == Operand::STAGED)
@@ -245,7 +245,7 @@ void KernelLauncher::release(Operand& op) {
 }
 
 void KernelLauncher::sync() {
-  if (ctx_) (void)hipStreamSynchronize(ctx_->stream());
+  if (ctx_) (void)wait_stream(ctx_->stream());
 }
 
 void KernelLauncher::set_profiling(bool on) {
