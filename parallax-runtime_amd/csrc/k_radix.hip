@@ -123,10 +123,44 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   const int wid = threadIdx.x / kWave;
   uint32_t* hw = h[wid];
   for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0;  // the wave's own copy
+  using U = typename RadixKey<T>::U;
   auto count = [&](T x) {
-    const auto k = RadixKey<T>::enc(x);
+    const U k = RadixKey<T>::enc(x);
 #pragma unroll
     for (int d = 0; d < D; ++d) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
+  };
+  // A 16-byte vector per lane: a digit position that the whole vector of every active
+  // lane shares is added once, by the first active lane (64 lanes adding to one LDS word
+  // serialise; narrow keys have such digits in most waves).  The test costs one OR of
+  // differences per key and one ballot per digit position.
+  auto count_vec = [&](const V& x) {
+    U k[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) k[j] = RadixKey<T>::enc(x.v[j]);
+    // (readfirstlane returns int: each half is taken as uint32_t before widening)
+    U ref;
+    if constexpr (sizeof(U) == 4) {
+      ref = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(k[0]));
+    } else {
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(k[0] >> 32)));
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(k[0])));
+      ref = (static_cast<U>(hi) << 32) | lo;
+    }
+    U diff = k[0] ^ ref;
+#pragma unroll
+    for (int j = 1; j < VEC; ++j) diff |= k[j] ^ ref;
+    const uint64_t active = __ballot(1);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (__ballot(((diff >> (8 * d)) & 0xffu) != 0u) == 0ull) {
+        if ((active & ((1ull << lane) - 1ull)) == 0ull)
+          atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(ref >> (8 * d)) & 0xffu)],
+                    static_cast<uint32_t>(VEC * __popcll(active)));
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k[j] >> (8 * d)) & 0xffu)], 1u);
+      }
+    }
   };
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * HB + threadIdx.x;
   const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * HB;
@@ -136,11 +170,15 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   if (gtid < head) count(data[gtid]);
   const V* vp = reinterpret_cast<const V*>(data + head);
   const uint64_t nv = (n - head) / VEC;
-  for (uint64_t v = gtid; v < nv; v += gstride) {
-    const V x = ld_stream(vp + v);
+  uint64_t v = gtid;
+  for (; v + 3 * gstride < nv; v += 4 * gstride) {
+    V xs[4];
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) count(x.v[j]);
+    for (int u = 0; u < 4; ++u) xs[u] = ld_stream(vp + v + u * gstride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) count_vec(xs[u]);
   }
+  for (; v < nv; v += gstride) count_vec(ld_stream(vp + v));
   const uint64_t body_end = head + nv * VEC;
   if (gtid < n - body_end) count(data[body_end + gtid]);
   __syncthreads();
@@ -161,7 +199,8 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   __threadfence();
 
   // Last block: exclusive bin starts of every digit position (one thread per bin);
-  // counters cleared.
+  // counters cleared; a position where one bin holds all n keys is marked skipped.
+  uint32_t skip = 0;
   for (int d = 0; d < D; ++d) {
     uint32_t c = 0, incl = 0;
     if (threadIdx.x < kBins) {
@@ -176,16 +215,22 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
       st_agent32(&ctl->radix_bins[d * kBins + threadIdx.x], off);
       st_agent32(&ctl->radix_hist[d * kBins + threadIdx.x], 0u);
     }
-    __syncthreads();
+    if (__syncthreads_or(threadIdx.x < kBins && c == static_cast<uint32_t>(n))) skip |= 1u << d;
   }
-  if (threadIdx.x == 0) st_agent32(&ctl->done, 0u);
+  if (threadIdx.x == 0) {
+    st_agent32(&ctl->radix_skip, skip);
+    st_agent32(&ctl->done, 0u);
+  }
 }
 
 // ---------------------------------------------------------------- one digit pass
 // Four waves per SIMD (128 VGPRs): 1024 threads per CU whatever the block size.
 //
-// SEG = false: one array [0, n); reads `a`, writes `b`; bin starts from the control
-// block.  SEG = true: the tiles of the long segments of a batch (RadixSegs tables; a
+// SEG = false: one array [0, n) in `a`, `b` a scratch array; bin starts from the control
+// block.  A digit that every key shares (marked by the histogram kernel) is skipped; the
+// k digits that run alternate a -> b -> a ..; when k is odd, the launch of the first
+// skipped digit copies the keys to the other array (an extra step in the alternation),
+// so the last step always writes `a`.  SEG = true: the tiles of the long segments of a batch (RadixSegs tables; a
 // tile never spans two segments); bin starts from the segment's own digit counts; a
 // tile looks back only over its segment's tiles; a digit that every key shares is
 // skipped (no block does anything).  The k digits that run move the keys between `a`
@@ -218,6 +263,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     auto dst = [&](int j) { return j == k - 1 ? 0 : ((k & 1) ? 1 + (j & 1) : 1 - (j & 1)); };
     in = buf[i == 0 ? (k == 1 ? 1 : 0) : dst(i - 1)];
     out = buf[dst(i)];
+  } else {
+    const uint32_t skip = ld_agent32(&ctl->radix_skip);
+    const int dig = shift / 8;
+    const uint32_t runs = ~skip & ((1u << D) - 1u);
+    const int k = __builtin_popcount(runs);  // odd k < D (D is even): skip != 0
+    const bool copy = (k & 1) && dig == __builtin_ctz(skip | (1u << D));
+    if (((skip >> dig) & 1u) && !copy) return;
+    const uint32_t below = (1u << dig) - 1u;
+    const int i = __builtin_popcount(runs & below) + (((k & 1) && (skip & below)) ? 1 : 0);
+    in = (i & 1) ? b : a;
+    out = (i & 1) ? a : b;
+    if (copy) {
+      constexpr int U = 8;  // loads in flight per thread
+      const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+      uint64_t x = static_cast<uint64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+      for (; x + (U - 1) * stride < n; x += U * stride) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = in[x + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) out[x + u * stride] = v[u];
+      }
+      for (; x < n; x += stride) out[x] = in[x];
+      return;
+    }
   }
 
   __shared__ uint32_t s_tile;
@@ -436,15 +506,13 @@ uint64_t radix_tile(size_t elem) { return static_cast<uint64_t>(radix_block()) *
 template <class T, int BLOCK>
 hipError_t radix_passes_t(T* data, T* tmp, uint64_t n, uint32_t grid, const Scratch& s,
                           hipStream_t stream, const LaunchHook& hook) {
-  T* a = data;
-  T* b = tmp;
+  // Every launch gets (array, scratch); each chooses its direction on the device.
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
     hook("sort_radix", 0);
-    k_radix_pass<T, BLOCK, false><<<grid, BLOCK, 0, stream>>>(a, b, n, 8 * d, RadixSegs{}, s.status, s.ctl);
+    k_radix_pass<T, BLOCK, false><<<grid, BLOCK, 0, stream>>>(data, tmp, n, 8 * d, RadixSegs{}, s.status, s.ctl);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hook("sort_radix", 1);
-    std::swap(a, b);
   }
   return hipSuccess;
 }
@@ -460,9 +528,10 @@ hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t 
   if (n >= (1ull << 32) || ntiles * kBins > s.status_words || s.partial_bytes < n * sizeof(T) || !tmp)
     return hipErrorInvalidValue;
   const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
-  // Histograms: 256-thread blocks, two per CU (measured best of 1, 2 and 4: the LDS
-  // atomics, not the loads, bound this kernel), no more blocks than 16 vectors per
-  // thread need.
+  // Histograms: 256-thread blocks, two per CU, no more blocks than 16 vectors per thread
+  // need.  The LDS atomics bound this kernel (about 21 cycles per wave-wide add with
+  // random digits): one block per CU or 1024-thread blocks run as fast; 3, 4 or 6 per
+  // CU are slower (each block adds D * 256 counters to the same global words).
   constexpr int HB = 256;
   const uint64_t hist_per_cu = tune().radix_hist_bpc > 0 ? static_cast<uint64_t>(tune().radix_hist_bpc) : 2;
   const uint64_t hist_work = (n + HB * 16ull * (16 / sizeof(T)) - 1) / (HB * 16ull * (16 / sizeof(T)));

@@ -198,6 +198,49 @@ def test_sort_radix_patterns(gpu_ctx, dt):
         assert got.tobytes() == pxo.sort(a).tobytes()
 
 
+def _shared_digit_keys(dt, kind, n, rng):
+    """Keys whose encoded radix digits are constant at some positions (those digit passes
+    are skipped; with an odd number left the first skipped one copies the keys)."""
+    if kind == "byte1":  # only digit 1 varies: one pass, copy before it
+        return (rng.integers(0, 256, n, dtype=np.uint64) << 8).astype(NP_DTYPES[dt])
+    if kind == "low24":  # top digit constant: three passes, copy after them
+        return rng.integers(0, 1 << 20, n).astype(NP_DTYPES[dt])
+    if kind == "mid":  # digit 2 constant: copy between passes
+        x = rng.integers(0, 1 << 32, n, dtype=np.uint64)
+        return ((x & 0xFF00FFFF) | 0x5A0000).astype(NP_DTYPES[dt])
+    if kind == "low32":  # 8-byte keys below 2^32: four of eight passes
+        return rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(NP_DTYPES[dt])
+    if kind == "low40":  # five of eight passes
+        return rng.integers(0, 1 << 40, n, dtype=np.uint64).astype(NP_DTYPES[dt])
+    if kind == "top":  # only the top digit varies
+        return (rng.integers(0, 128, n, dtype=np.uint64) << 56).astype(NP_DTYPES[dt])
+    if kind == "unit":  # floats in [1, 2): sign and exponent constant
+        return (1.0 + rng.random(n)).astype(NP_DTYPES[dt])
+    if kind == "ints":  # small non-negative integers as floats: low mantissa digits zero
+        return rng.integers(0, 256, n).astype(NP_DTYPES[dt])
+    raise ValueError(kind)
+
+
+SHARED_DIGIT_CASES = [(PX_U32, "byte1"), (PX_U32, "low24"), (PX_U32, "mid"), (PX_I32, "low24"),
+                      (PX_U64, "low32"), (PX_I64, "low40"), (PX_U64, "top"), (PX_U64, "byte1"),
+                      (PX_F32, "unit"), (PX_F64, "unit"), (PX_F64, "ints"), (PX_F32, "ints")]
+
+
+@pytest.mark.parametrize("dt,kind", SHARED_DIGIT_CASES)
+def test_sort_radix_shared_digits(gpu_ctx, dt, kind):
+    """Radix sorts that skip the digit positions every key shares, on an odd and an
+    even count of remaining passes, and twice in a row on one stream."""
+    n = (1 << 20) + 7
+    rng = np.random.default_rng(31 + dt)
+    for rep in range(2):
+        a = _shared_digit_keys(dt, kind, n + rep, rng)
+        b, p = upload(a, rep)
+        check(lib.pxd_sort(gpu_ctx.h, dt, p, n + rep), "sort")
+        gpu_ctx.sync()
+        got = b.download(a.dtype, n + rep, rep * a.itemsize)
+        assert got.tobytes() == pxo.sort(a).tobytes(), (kind, rep)
+
+
 def test_sort_radix_signed_zero_nan(gpu_ctx):
     """-0.0 / +0.0 are equal under operator<, NaNs unordered: the radix order puts -0.0
     before +0.0 and NaNs by sign at the ends; everything else matches std::sort."""
