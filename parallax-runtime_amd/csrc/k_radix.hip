@@ -96,11 +96,15 @@ __device__ __forceinline__ uint32_t digit_of(T x, int shift) {
   return static_cast<uint32_t>(RadixKey<T>::enc(x) >> shift) & 0xffu;
 }
 
-// Keys per lane: 16 for every key size (1024-thread tiles of 16K keys; for 8-byte keys
-// 128 KiB of LDS).  Measured: 8-byte keys at 8 per lane 1.85 ms, at 16 per lane
-// 1.51 ms (2^25 f64); 4-byte keys at 24 or 32 per lane spill.
+// Keys per lane of a whole-array digit pass: 16 for 8-byte keys (1024-thread tiles of 16K
+// keys, 128 KiB of LDS), 20 for 4-byte keys (the most without spills: 22 spills 6
+// VGPRs, 24 spills 17).  Measured: 8-byte keys at 8 per lane 1.85 ms, at 16 per lane
+// 1.51 ms (2^25 f64); 4-byte keys at 16 / 18 / 20 per lane 1.08 / 1.06 / 1.02 ms (2^26
+// u32: fewer tiles, fewer look-backs).  The segmented pass keeps 16 (at 20 it spills).
 template <class T>
-constexpr int kKpt = 16;
+constexpr int kKpt = sizeof(T) == 4 ? 20 : 16;
+template <class T>
+constexpr int kKptSeg = 16;
 
 __device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t state, uint32_t v) {
   return (static_cast<uint64_t>((epoch << 2) | state) << 32) | v;
@@ -242,8 +246,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     T* __restrict__ a, T* __restrict__ b, uint64_t n, int shift, RadixSegs g,
     uint64_t* status, Ctl* ctl) {
   static_assert(BLOCK >= kBins, "one thread per digit in the count phases");
-  static_assert(BLOCK * kKpt<T> <= 65536, "ranks are kept in 16 bits");
-  constexpr int KPT = kKpt<T>;
+  constexpr int KPT = SEG ? kKptSeg<T> : kKpt<T>;
+  static_assert(BLOCK * KPT <= 65536, "ranks are kept in 16 bits");
+  static_assert(KPT % 2 == 0, "ranks are packed two per register");
   constexpr int NW = BLOCK / kWave;
   constexpr int WSPAN = KPT * kWave;
   constexpr int TILE = BLOCK * KPT;
@@ -564,7 +569,7 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n) {
 template <class T>
 __device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
                                          uint32_t& tbase, uint32_t& tend) {
-  constexpr uint32_t TILE = kSegBlock * kKpt<T>;
+  constexpr uint32_t TILE = kSegBlock * kKptSeg<T>;
   seg = g.tile_seg[tile];
   const uint32_t lo = static_cast<uint32_t>(g.seg_lo[seg]);
   const uint32_t hi = static_cast<uint32_t>(g.seg_hi[seg]);
@@ -770,7 +775,7 @@ uint64_t seg_radix_min_len(int dtype) {
 }
 
 uint32_t seg_radix_tile(int dtype) {
-  return static_cast<uint32_t>(kSegBlock * (dtype_size(dtype) == 4 ? kKpt<uint32_t> : kKpt<uint64_t>));
+  return static_cast<uint32_t>(kSegBlock * (dtype_size(dtype) == 4 ? kKptSeg<uint32_t> : kKptSeg<uint64_t>));
 }
 
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg) {

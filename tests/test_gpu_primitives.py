@@ -154,7 +154,8 @@ def test_sort_extremes(gpu_ctx):
 
 
 
-RADIX_SIZES = [1 << 20, (1 << 20) + 3, 3 * (1 << 20) + 7, (1 << 22) + 1]
+# whole 16K-key (8-byte) and 20K-key (4-byte) tiles, partial last tiles
+RADIX_SIZES = [1 << 20, 64 * 20480, (1 << 20) + 3, 3 * (1 << 20) + 7, (1 << 22) + 1]
 
 
 @pytest.mark.parametrize("dt", ALL)
