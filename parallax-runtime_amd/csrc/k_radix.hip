@@ -111,6 +111,38 @@ __device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t state, 
 }
 
 // ---------------------------------------------------------------- histograms
+// Adds N encoded keys per lane to a wave's LDS histograms hw[D * 256].  A digit position
+// that all N keys of every active lane share is added once, by the first active lane (64
+// lanes adding to one LDS word serialise; narrow keys have such digits in most waves);
+// the test costs one OR of differences per key and one ballot per digit position.
+template <int D, int N, class U>
+__device__ __forceinline__ void hist_add(uint32_t* hw, const U (&k)[N], int lane) {
+  U ref;  // the first active lane's first key (readfirstlane returns int: each half is
+          // taken as uint32_t before widening)
+  if constexpr (sizeof(U) == 4) {
+    ref = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(k[0]));
+  } else {
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(k[0] >> 32)));
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(k[0])));
+    ref = (static_cast<U>(hi) << 32) | lo;
+  }
+  U diff = k[0] ^ ref;
+#pragma unroll
+  for (int j = 1; j < N; ++j) diff |= k[j] ^ ref;
+  const uint64_t active = __ballot(1);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (__ballot(((diff >> (8 * d)) & 0xffu) != 0u) == 0ull) {
+      if ((active & ((1ull << lane) - 1ull)) == 0ull)
+        atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(ref >> (8 * d)) & 0xffu)],
+                  static_cast<uint32_t>(N * __popcll(active)));
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k[j] >> (8 * d)) & 0xffu)], 1u);
+    }
+  }
+}
+
 // Each wave counts into its own LDS histograms (no contention between waves); the
 // block merges them and adds the totals to the control block's counters.
 template <class T, int HB>
@@ -133,39 +165,6 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 #pragma unroll
     for (int d = 0; d < D; ++d) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
   };
-  // A 16-byte vector per lane: a digit position that the whole vector of every active
-  // lane shares is added once, by the first active lane (64 lanes adding to one LDS word
-  // serialise; narrow keys have such digits in most waves).  The test costs one OR of
-  // differences per key and one ballot per digit position.
-  auto count_vec = [&](const V& x) {
-    U k[VEC];
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) k[j] = RadixKey<T>::enc(x.v[j]);
-    // (readfirstlane returns int: each half is taken as uint32_t before widening)
-    U ref;
-    if constexpr (sizeof(U) == 4) {
-      ref = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(k[0]));
-    } else {
-      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(k[0] >> 32)));
-      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(k[0])));
-      ref = (static_cast<U>(hi) << 32) | lo;
-    }
-    U diff = k[0] ^ ref;
-#pragma unroll
-    for (int j = 1; j < VEC; ++j) diff |= k[j] ^ ref;
-    const uint64_t active = __ballot(1);
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      if (__ballot(((diff >> (8 * d)) & 0xffu) != 0u) == 0ull) {
-        if ((active & ((1ull << lane) - 1ull)) == 0ull)
-          atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(ref >> (8 * d)) & 0xffu)],
-                    static_cast<uint32_t>(VEC * __popcll(active)));
-      } else {
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k[j] >> (8 * d)) & 0xffu)], 1u);
-      }
-    }
-  };
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * HB + threadIdx.x;
   const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * HB;
   const uint32_t mis = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(data) & 15u) / sizeof(T));
@@ -174,15 +173,26 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   if (gtid < head) count(data[gtid]);
   const V* vp = reinterpret_cast<const V*>(data + head);
   const uint64_t nv = (n - head) / VEC;
+  constexpr int UL = 4;  // vector loads in flight per thread; one digit test per group
   uint64_t v = gtid;
-  for (; v + 3 * gstride < nv; v += 4 * gstride) {
-    V xs[4];
+  for (; v + (UL - 1) * gstride < nv; v += UL * gstride) {
+    V xs[UL];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xs[u] = ld_stream(vp + v + u * gstride);
+    for (int u = 0; u < UL; ++u) xs[u] = ld_stream(vp + v + u * gstride);
+    U k[UL * VEC];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) count_vec(xs[u]);
+    for (int u = 0; u < UL; ++u)
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) k[u * VEC + j] = RadixKey<T>::enc(xs[u].v[j]);
+    hist_add<D>(hw, k, lane);
   }
-  for (; v < nv; v += gstride) count_vec(ld_stream(vp + v));
+  for (; v < nv; v += gstride) {
+    const V x = ld_stream(vp + v);
+    U k[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) k[j] = RadixKey<T>::enc(x.v[j]);
+    hist_add<D>(hw, k, lane);
+  }
   const uint64_t body_end = head + nv * VEC;
   if (gtid < n - body_end) count(data[body_end + gtid]);
   __syncthreads();
@@ -227,6 +237,21 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   }
 }
 
+// ---------------------------------------------------------------- segment geometry
+constexpr int kSegBlock = 1024;  // digit-pass block of the segmented sort
+
+// Tile t of the long segments covers [tbase, tend).
+template <class T>
+__device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
+                                         uint32_t& tbase, uint32_t& tend) {
+  constexpr uint32_t TILE = kSegBlock * kKptSeg<T>;
+  seg = g.tile_seg[tile];
+  const uint32_t lo = static_cast<uint32_t>(g.seg_lo[seg]);
+  const uint32_t hi = static_cast<uint32_t>(g.seg_hi[seg]);
+  tbase = lo + (tile - g.seg_first[seg]) * TILE;
+  tend = hi - tbase < TILE ? hi : tbase + TILE;
+}
+
 // ---------------------------------------------------------------- one digit pass
 // Four waves per SIMD (128 VGPRs): 1024 threads per CU whatever the block size.
 //
@@ -240,7 +265,8 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 // skipped (no block does anything).  The k digits that run move the keys between `a`
 // (the array), `b` and g.tmp2 so that the last one writes the array and none reads and
 // writes the same buffer: k even a -> b -> a ..; k odd (>= 3) a -> b -> tmp2 -> b ..
-// -> a; k = 1 b -> a (the histogram kernel copied the keys to b).
+// -> a; k = 1: that digit's pass and the first skipped digit's launch, which copies the
+// long segments' keys, go a -> b -> a in launch order.
 template <class T, int BLOCK, bool SEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_radix_pass(
     T* __restrict__ a, T* __restrict__ b, uint64_t n, int shift, RadixSegs g,
@@ -260,14 +286,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   if constexpr (SEG) {
     const uint32_t skip = ld_agent32(&ctl->radix_skip);
     const int dig = shift / 8;
-    if ((skip >> dig) & 1u) return;  // every key shares this digit
     const uint32_t runs = ~skip & ((1u << D) - 1u);
     const int k = __builtin_popcount(runs);
-    const int i = __builtin_popcount(runs & ((1u << dig) - 1u));  // this pass's run index
-    T* const buf[3] = {a, b, static_cast<T*>(g.tmp2)};
-    auto dst = [&](int j) { return j == k - 1 ? 0 : ((k & 1) ? 1 + (j & 1) : 1 - (j & 1)); };
-    in = buf[i == 0 ? (k == 1 ? 1 : 0) : dst(i - 1)];
-    out = buf[dst(i)];
+    if (k == 1) {
+      const int r = __builtin_ctz(runs);
+      const int s0 = __builtin_ctz(skip);  // D > 1, so some digit is skipped
+      if (dig != r && dig != s0) return;
+      const bool first = dig == (r < s0 ? r : s0);
+      in = first ? a : b;
+      out = first ? b : a;
+      if (dig == s0) {
+        for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+          uint32_t sg, tb, te;
+          seg_tile<T>(g, t, sg, tb, te);
+          for (uint32_t p = tb + threadIdx.x; p < te; p += BLOCK) out[p] = in[p];
+        }
+        return;
+      }
+    } else {
+      if ((skip >> dig) & 1u) return;  // every key shares this digit
+      const int i = __builtin_popcount(runs & ((1u << dig) - 1u));  // this pass's run index
+      T* const buf[3] = {a, b, static_cast<T*>(g.tmp2)};
+      auto dst = [&](int j) { return j == k - 1 ? 0 : ((k & 1) ? 1 + (j & 1) : 1 - (j & 1)); };
+      in = buf[i == 0 ? 0 : dst(i - 1)];
+      out = buf[dst(i)];
+    }
   } else {
     const uint32_t skip = ld_agent32(&ctl->radix_skip);
     const int dig = shift / 8;
@@ -556,75 +599,119 @@ hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t 
 }
 
 // ---------------------------------------------------------------- segmented
-constexpr int kSegBlock = 1024;  // digit-pass block of the segmented sort
-constexpr int kSegUnroll = 8;    // loads in flight per thread in the prologue kernels
-
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * 256)
     p[i] = 0u;
 }
 
-// Tile t of the long segments covers [tbase, tend).
+// Digit counts per long segment, and which digits every key shares.  Each block takes a
+// contiguous range of tiles and keeps counting in LDS (a copy per wave) while
+// consecutive tiles belong to the same segment; it adds the counts to the segment's
+// totals when the segment changes and at the end (agent-scope adds, since a segment's
+// tiles may span blocks).  Keys are read as 16-byte vectors (a tile's unaligned head
+// and tail one key at a time); a digit position that a whole vector of every active
+// lane shares is added once by one lane.  The same pass ORs the encoded keys and their
+// complements; the last block marks the digits every key shares (their passes are
+// skipped) and clears the ORs for the next call.
 template <class T>
-__device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
-                                         uint32_t& tbase, uint32_t& tend) {
-  constexpr uint32_t TILE = kSegBlock * kKptSeg<T>;
-  seg = g.tile_seg[tile];
-  const uint32_t lo = static_cast<uint32_t>(g.seg_lo[seg]);
-  const uint32_t hi = static_cast<uint32_t>(g.seg_hi[seg]);
-  tbase = lo + (tile - g.seg_first[seg]) * TILE;
-  tend = hi - tbase < TILE ? hi : tbase + TILE;
-}
-
-// OR of the encoded keys and of their complements over the long segments (16-byte
-// loads; a tile's unaligned head and tail one key at a time).  The last block marks the
-// digits every key shares (their passes are skipped), sets the selector when exactly
-// one digit runs (the histogram kernel then copies the keys to the scratch copy, which
-// that pass reads) and clears the ORs for the next call.
-template <class T>
-__global__ __launch_bounds__(256) void k_segradix_or(const T* data, RadixSegs g, Ctl* ctl) {
+__global__ __launch_bounds__(256) void k_segradix_hist(const T* data, RadixSegs g, Ctl* ctl) {
   constexpr int D = sizeof(T);
+  constexpr int HB = 256;
+  constexpr int NWH = HB / kWave;
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int UL = 4;  // vector loads in flight per thread
   using U = typename RadixKey<T>::U;
+  using V = Vec<T, VEC>;
+  __shared__ uint32_t h[NWH][D * kBins];
   __shared__ unsigned long long s_or, s_nor;
   __shared__ int s_last;
+
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  uint32_t* hw = h[wid];
   if (threadIdx.x == 0) {
     s_or = 0ull;
     s_nor = 0ull;
   }
+  for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;  // the wave's own copy
   __syncthreads();
-  constexpr int VEC = 16 / sizeof(T);
-  using V = Vec<T, VEC>;
   U acc_or = 0, acc_nor = 0;
-  auto add = [&](T x) {
+  auto count1 = [&](T x) {
     const U k = RadixKey<T>::enc(x);
     acc_or |= k;
     acc_nor |= static_cast<U>(~k);
+#pragma unroll
+    for (int d = 0; d < D; ++d) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
   };
-  for (uint32_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
+  auto count_keys = [&](auto& k) {  // encoded keys of this lane
+#pragma unroll
+    for (auto& x : k) {
+      acc_or |= x;
+      acc_nor |= static_cast<U>(~x);
+    }
+    hist_add<D>(hw, k, lane);
+  };
+  auto flush = [&](uint32_t seg) {
+    __syncthreads();
+    uint32_t* cnt = g.seg_cnt + static_cast<uint64_t>(seg) * D * kBins;
+    for (int i = threadIdx.x; i < D * kBins; i += HB) {
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < NWH; ++w) c += h[w][i];
+      if (c) __hip_atomic_fetch_add(&cnt[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;
+  };
+  // gridDim.x <= g.ntiles: every block has at least one tile
+  const uint32_t t0 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * g.ntiles / gridDim.x);
+  const uint32_t t1 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x + 1) * g.ntiles / gridDim.x);
+  uint32_t cur = 0xffffffffu;
+  for (uint32_t tile = t0; tile < t1; ++tile) {
     uint32_t seg, tbase, tend;
     seg_tile<T>(g, tile, seg, tbase, tend);
+    if (seg != cur) {
+      if (cur != 0xffffffffu) flush(cur);
+      cur = seg;
+    }
     const uint32_t mis = static_cast<uint32_t>(((reinterpret_cast<uintptr_t>(data + tbase)) & 15u) / sizeof(T));
     uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
     if (head > tend - tbase) head = tend - tbase;
     const uint32_t nvec = (tend - tbase - head) / VEC;
     const uint32_t body_end = tbase + head + nvec * VEC;
-    if (threadIdx.x < head) add(data[tbase + threadIdx.x]);
-    if (threadIdx.x < tend - body_end) add(data[body_end + threadIdx.x]);
+    if (threadIdx.x < head) count1(data[tbase + threadIdx.x]);
+    if (threadIdx.x < tend - body_end) count1(data[body_end + threadIdx.x]);
     const V* vp = reinterpret_cast<const V*>(data + tbase + head);
-    for (uint32_t v0 = threadIdx.x; v0 < nvec; v0 += 256 * kSegUnroll) {
-      V x[kSegUnroll];  // independent loads in flight
+    for (uint32_t v0 = threadIdx.x; v0 < nvec; v0 += HB * UL) {
+      V x[UL];
 #pragma unroll
-      for (int u = 0; u < kSegUnroll; ++u) {
-        const uint32_t v = v0 + u * 256;
-        x[u] = ld_stream(vp + (v < nvec ? v : v0));
+      for (int u = 0; u < UL; ++u) {
+        const uint32_t v = v0 + u * HB;
+        if (v < nvec) x[u] = ld_stream(vp + v);
       }
+      if (v0 + (UL - 1) * HB < nvec) {  // a whole group: one digit test for UL vectors
+        U k[UL * VEC];
 #pragma unroll
-      for (int u = 0; u < kSegUnroll; ++u)
+        for (int u = 0; u < UL; ++u)
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) add(x[u].v[j]);
+          for (int j = 0; j < VEC; ++j) k[u * VEC + j] = RadixKey<T>::enc(x[u].v[j]);
+        count_keys(k);
+      } else {
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+          if (v0 + u * HB < nvec) {
+            U k[VEC];
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) k[j] = RadixKey<T>::enc(x[u].v[j]);
+            count_keys(k);
+          }
+        }
+      }
     }
   }
+  if (cur != 0xffffffffu) flush(cur);
+
   atomicOr(&s_or, static_cast<unsigned long long>(acc_or));
   atomicOr(&s_nor, static_cast<unsigned long long>(acc_nor));
   __syncthreads();
@@ -649,78 +736,11 @@ __global__ __launch_bounds__(256) void k_segradix_or(const T* data, RadixSegs g,
     st_agent(&ctl->radix_nor[q], 0ull);
   }
   const uint64_t varying = all_or & all_nor;  // set in some key, clear in another
-  uint32_t skip = 0, runs = 0;
-  for (int d = 0; d < D; ++d) {
+  uint32_t skip = 0;
+  for (int d = 0; d < D; ++d)
     if (((varying >> (8 * d)) & 0xffu) == 0u) skip |= 1u << d;
-    else ++runs;
-  }
   st_agent32(&ctl->radix_skip, skip);
-  st_agent32(&ctl->radix_sel, runs == 1u ? 1u : 0u);
   st_agent32(&ctl->done, 0u);
-}
-
-// Counts of the digits that run, per long segment.  Each block takes a contiguous range
-// of tiles and keeps counting in LDS (a copy per wave) while consecutive tiles belong to
-// the same segment; it adds the counts to the segment's totals when the segment changes
-// and at the end (agent-scope adds, since a segment's tiles may span blocks).  When
-// exactly one digit runs the keys are also copied to the scratch copy here.
-template <class T>
-__global__ __launch_bounds__(256) void k_segradix_hist(const T* data, T* tmp, RadixSegs g, Ctl* ctl) {
-  constexpr int D = sizeof(T);
-  constexpr int HB = 256;
-  constexpr int NWH = HB / kWave;
-  using U = typename RadixKey<T>::U;
-  __shared__ uint32_t h[NWH][D * kBins];
-
-  const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
-  uint32_t* hw = h[wid];
-  const uint32_t skip = ld_agent32(&ctl->radix_skip);
-  const bool copy = ld_agent32(&ctl->radix_sel) != 0u;
-  const uint32_t t0 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * g.ntiles / gridDim.x);
-  const uint32_t t1 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x + 1) * g.ntiles / gridDim.x);
-  auto flush = [&](uint32_t seg) {
-    __syncthreads();
-    uint32_t* cnt = g.seg_cnt + static_cast<uint64_t>(seg) * D * kBins;
-    for (int i = threadIdx.x; i < D * kBins; i += HB) {
-      uint32_t c = 0;
-#pragma unroll
-      for (int w = 0; w < NWH; ++w) c += h[w][i];
-      if (c) __hip_atomic_fetch_add(&cnt[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;
-  };
-  for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;  // the wave's own copy
-  uint32_t cur = 0xffffffffu;
-  for (uint32_t tile = t0; tile < t1; ++tile) {
-    uint32_t seg, tbase, tend;
-    seg_tile<T>(g, tile, seg, tbase, tend);
-    if (seg != cur) {
-      if (cur != 0xffffffffu) flush(cur);
-      cur = seg;
-    }
-    for (uint32_t p0 = tbase + threadIdx.x; p0 < tend; p0 += HB * kSegUnroll) {
-      T x[kSegUnroll];  // independent loads in flight
-#pragma unroll
-      for (int u = 0; u < kSegUnroll; ++u) {
-        const uint32_t p = p0 + u * HB;
-        if (p < tend) x[u] = data[p];
-      }
-#pragma unroll
-      for (int u = 0; u < kSegUnroll; ++u) {
-        const uint32_t p = p0 + u * HB;
-        if (p < tend) {
-          if (copy) tmp[p] = x[u];
-          const U k = RadixKey<T>::enc(x[u]);
-#pragma unroll
-          for (int d = 0; d < D; ++d)
-            if (!((skip >> d) & 1u)) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
-        }
-      }
-    }
-  }
-  if (cur != 0xffffffffu) flush(cur);
 }
 
 template <class T>
@@ -737,11 +757,9 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
   k_fill_u32<<<static_cast<uint32_t>(std::min<uint64_t>((ncnt + 255) / 256, 1024)), 256, 0, stream>>>(g.seg_cnt, ncnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint32_t ogrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 2ull * cus));
-  k_segradix_or<T><<<ogrid, 256, 0, stream>>>(data, g, s.ctl);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 4ull * cus));
-  k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, tmp, g, s.ctl);
+  const uint64_t hist_per_cu = tune().seg_hist_bpc > 0 ? static_cast<uint64_t>(tune().seg_hist_bpc) : 4;
+  const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, hist_per_cu * cus));
+  k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, g, s.ctl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, cus));
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {

@@ -26,8 +26,7 @@ struct Ctl {
   uint64_t result;  // kept count written by the last compaction tile
   uint64_t pad1[5];
   uint8_t value[64];  // reduce result slot (one element, written by the last block)
-  uint32_t radix_sel;   // segmented radix: 1 when one digit runs (keys copied to scratch)
-  uint32_t radix_skip;  // segmented radix: bit d set when every key shares digit d
+  uint32_t radix_skip;  // radix sorts: bit d set when every key shares digit d
   uint64_t radix_or[16];   // segmented radix: OR of the encoded keys, 16 slots (block % 16)
   uint64_t radix_nor[16];  // ... OR of their complements; all zero between calls
   uint32_t radix_hist[8 * 256];  // radix sort digit counts (zero between calls)
@@ -61,6 +60,7 @@ struct Tune {
   uint64_t sort_radix_min = 1ull << 20;  // plain sorts of at least this many keys use radix (0: never)
   int radix_block = 1024;     // threads per radix digit-pass block (256, 512, 1024)
   int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 2)
+  int seg_hist_bpc = 0;       // segmented radix histogram blocks per CU (0: default, 4)
 };
 Tune& tune();
 // Compiled tile shapes of the look-back kernels: op PX_OP_SCAN (scans) or PX_OP_FLAGS

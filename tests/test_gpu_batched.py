@@ -102,14 +102,21 @@ def radix_seg_input(dt, seed, kind):
         else:
             info = np.iinfo(t)
             a = rng.integers(info.min, info.max, n, dtype=t, endpoint=True)
-    elif kind == "narrow":  # 20 significant bits: three digits run, a copy back follows
-        a = rng.integers(0, 1 << 20, n).astype(t) if dt not in FLOATS else rng.integers(0, 1 << 20, n).astype(t)
-    else:  # "equal": long segments of one value each (every digit skipped) + short random ones
+    elif kind == "narrow":  # 20 significant bits: three digits run (integers)
+        a = rng.integers(0, 1 << 20, n).astype(t)
+    elif kind == "byte":  # one digit varies: one pass plus one copy (integers: pass first)
+        if dt == PX_F32:  # 1 + j/128: only digit 2 varies
+            a = (1.0 + rng.integers(0, 128, n) / 128.0).astype(t)
+        elif dt == PX_F64:  # 1 + j/16: only digit 6 varies
+            a = (1.0 + rng.integers(0, 16, n) / 16.0).astype(t)
+        else:
+            a = rng.integers(0, 256, n).astype(t)
+    else:  # "equal": each segment holds one value (values below 1000 across segments)
         a = np.repeat(rng.integers(0, 1000, lens.size).astype(t), lens)
     return a, off
 
 
-@pytest.mark.parametrize("kind", ["full", "narrow", "equal"])
+@pytest.mark.parametrize("kind", ["full", "narrow", "byte", "equal"])
 @pytest.mark.parametrize("dt", ALL)
 def test_seg_sort_radix(gpu_ctx, dt, kind):
     a, off = radix_seg_input(dt, 60 + dt, kind)
@@ -117,7 +124,7 @@ def test_seg_sort_radix(gpu_ctx, dt, kind):
     check(lib.pxb_seg_sort(gpu_ctx.h, dt, d.ptr, u64p(off), off.size - 1), "seg_sort")
     gpu_ctx.sync()
     assert d.download(a.dtype, a.size).tobytes() == pxo.seg_sort(a, off).tobytes()
-    # again on the same context (scratch, counters and selector reused)
+    # again on the same context (scratch, counters and skip mask reused)
     b = a[::-1].copy()
     d2 = DeviceBuffer.from_array(b)
     check(lib.pxb_seg_sort(gpu_ctx.h, dt, d2.ptr, u64p(off), off.size - 1), "seg_sort")
