@@ -771,12 +771,18 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
 
 }  // namespace
 
-uint64_t radix_min() {
-  return tune().sort_radix_min;
+// Radix below these sizes loses to the bitonic path: each digit pass is at least one
+// tile's serial load, rank and write on one CU (about 10 us).  Measured (2^k keys,
+// bitonic / radix): u32 2^17 72 / 77 us, 2^18 85 / 81, 2^19 103 / 86; f64 2^18
+// 112 / 157, 2^19 144 / 172, 2^20 203 / 202.
+uint64_t radix_min(int dtype) {
+  const uint64_t t = tune().sort_radix_min;
+  if (t != ~0ull) return t;
+  return dtype_size(dtype) == 4 ? (1ull << 18) : (1ull << 20);
 }
 
 bool radix_sort_ok(int dtype, uint64_t n) {
-  const uint64_t lo = radix_min();
+  const uint64_t lo = radix_min(dtype);
   return dtype_size(dtype) > 0 && lo > 0 && n >= lo && n < (1ull << 32) - (1ull << 20);
 }
 

@@ -177,6 +177,18 @@ def test_sort_radix(gpu_ctx, dt, n):
     assert got.tobytes() == pxo.sort(a).tobytes()
 
 
+@pytest.mark.parametrize("dt", [PX_I32, PX_U32, PX_F32])
+@pytest.mark.parametrize("n", [1 << 18, (1 << 18) + 5, 700_001, (1 << 20) - 1])
+def test_sort_radix_4byte_from_2p18(gpu_ctx, dt, n):
+    """4-byte keys take the radix path from 2^18 keys (8-byte ones from 2^20)."""
+    a = gen(dt, n, 950 + n % 89 + dt)
+    a[::3] = a[1]  # duplicates
+    b, p = upload(a, n % 2)
+    check(lib.pxd_sort(gpu_ctx.h, dt, p, n), "sort")
+    gpu_ctx.sync()
+    assert b.download(a.dtype, n, (n % 2) * a.itemsize).tobytes() == pxo.sort(a).tobytes()
+
+
 @pytest.mark.parametrize("dt", ALL)
 def test_sort_radix_patterns(gpu_ctx, dt):
     """All-equal, sorted, reversed and extreme keys through the radix path."""
