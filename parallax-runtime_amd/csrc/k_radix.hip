@@ -544,12 +544,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 }
 
 // Threads per digit-pass block (256, 512 or 1024); blocks per CU = 1024 / block.
-int radix_block() {
+// Smaller arrays take smaller blocks: more tiles, so more CUs share each pass (measured
+// for 2^k u32 keys, 1024 / 512 / 256 threads: 2^17 77 / 68 / 65 us, 2^19 86 / 80 / 86,
+// 2^20 98 / 98 / 113); 8-byte keys take the radix path only from 2^20, where 1024 is
+// best.
+int radix_block(size_t elem, uint64_t n) {
   const int b = tune().radix_block;
-  return (b == 256 || b == 512 || b == 1024) ? b : 1024;
+  if (b == 256 || b == 512 || b == 1024) return b;
+  if (elem == 4 && n < (1ull << 19)) return 256;
+  if (elem == 4 && n < (1ull << 20)) return 512;
+  return 1024;
 }
 
-uint64_t radix_tile(size_t elem) { return static_cast<uint64_t>(radix_block()) * (elem == 4 ? kKpt<uint32_t> : kKpt<uint64_t>); }
+uint64_t radix_tile(size_t elem, uint64_t n) {
+  return static_cast<uint64_t>(radix_block(elem, n)) * (elem == 4 ? kKpt<uint32_t> : kKpt<uint64_t>);
+}
 
 template <class T, int BLOCK>
 hipError_t radix_passes_t(T* data, T* tmp, uint64_t n, uint32_t grid, const Scratch& s,
@@ -570,8 +579,8 @@ hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t 
                         const LaunchHook& hook) {
   T* data = static_cast<T*>(data_v);
   T* tmp = static_cast<T*>(s.partials);
-  const int block = radix_block();
-  const uint64_t TILE = radix_tile(sizeof(T));
+  const int block = radix_block(sizeof(T), n);
+  const uint64_t TILE = radix_tile(sizeof(T), n);
   const uint64_t ntiles = (n + TILE - 1) / TILE;
   if (n >= (1ull << 32) || ntiles * kBins > s.status_words || s.partial_bytes < n * sizeof(T) || !tmp)
     return hipErrorInvalidValue;
@@ -773,12 +782,13 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
 
 // Radix below these sizes loses to the bitonic path: each digit pass is at least one
 // tile's serial load, rank and write on one CU (about 10 us).  Measured (2^k keys,
-// bitonic / radix): u32 2^17 72 / 77 us, 2^18 85 / 81, 2^19 103 / 86; f64 2^18
-// 112 / 157, 2^19 144 / 172, 2^20 203 / 202.
+// bitonic / radix with the block chosen by radix_block): u32 2^16 61 / 62 us, 2^17
+// 72 / 65, 2^18 85 / 70, 2^19 103 / 80; f64 2^18 112 / 134, 2^19 144 / 157, 2^20
+// 203 / 202.
 uint64_t radix_min(int dtype) {
   const uint64_t t = tune().sort_radix_min;
   if (t != ~0ull) return t;
-  return dtype_size(dtype) == 4 ? (1ull << 18) : (1ull << 20);
+  return dtype_size(dtype) == 4 ? (1ull << 17) : (1ull << 20);
 }
 
 bool radix_sort_ok(int dtype, uint64_t n) {
@@ -787,7 +797,7 @@ bool radix_sort_ok(int dtype, uint64_t n) {
 }
 
 uint64_t radix_status_words(int dtype, uint64_t n) {
-  const uint64_t tile = radix_tile(dtype_size(dtype));
+  const uint64_t tile = radix_tile(dtype_size(dtype), n);
   return ((n + tile - 1) / tile) * kBins;
 }
 

@@ -57,10 +57,10 @@ struct Tune {
   int red_nt = 1;             // non-temporal loads in the plain reduce
   int sort_c4 = 6;            // stages per HBM register pass of the sort, 4-byte keys (<= 7)
   int sort_c8 = 6;            // ... 8-byte keys (<= 6)
-  // plain sorts of at least this many keys use radix (0: never; ~0: 2^18 keys
+  // plain sorts of at least this many keys use radix (0: never; ~0: 2^17 keys
   // of 4 bytes, 2^20 of 8 bytes, the measured crossovers with the bitonic path)
   uint64_t sort_radix_min = ~0ull;
-  int radix_block = 1024;     // threads per radix digit-pass block (256, 512, 1024)
+  int radix_block = 0;        // threads per radix digit-pass block (256, 512, 1024; 0: by size)
   int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 2)
   int seg_hist_bpc = 0;       // segmented radix histogram blocks per CU (0: default, 4)
 };

@@ -178,9 +178,10 @@ def test_sort_radix(gpu_ctx, dt, n):
 
 
 @pytest.mark.parametrize("dt", [PX_I32, PX_U32, PX_F32])
-@pytest.mark.parametrize("n", [1 << 18, (1 << 18) + 5, 700_001, (1 << 20) - 1])
-def test_sort_radix_4byte_from_2p18(gpu_ctx, dt, n):
-    """4-byte keys take the radix path from 2^18 keys (8-byte ones from 2^20)."""
+@pytest.mark.parametrize("n", [1 << 17, (1 << 17) + 3, 1 << 18, (1 << 18) + 5, 700_001, (1 << 20) - 1])
+def test_sort_radix_4byte_below_2p20(gpu_ctx, dt, n):
+    """4-byte keys take the radix path from 2^17 keys (256-thread digit-pass blocks below
+    2^19, 512 below 2^20); 8-byte ones from 2^20."""
     a = gen(dt, n, 950 + n % 89 + dt)
     a[::3] = a[1]  # duplicates
     b, p = upload(a, n % 2)
