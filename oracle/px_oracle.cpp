@@ -15,6 +15,14 @@
 //   count_if      stdpar.hpp:290-292
 //   for_each map  stdpar.hpp:81        f(data[i]) in place
 //   transform     stdpar.hpp:119       out[i] = f(in[i])
+// The search entry points have no host loop in stdpar.hpp; their semantics are the
+// reference launcher's host combine over per-block winners:
+//   find          kernel_launcher.cpp:1180-1189  first index whose element passes; n if none
+//   mismatch      kernel_launcher.cpp:1272-1277  first index with a[i] != b[i]; n if none
+//   argminmax     kernel_launcher.cpp:1078-1098  min (max) by <, ties to the smaller index
+//                 (the larger with want_last); 0 for n == 0.  Values compare in the
+//                 element type here (the reference widens them to double, which merges
+//                 int64 values that differ beyond 2^53).
 // Integer arithmetic is done in the unsigned type of the same width, which is the
 // two's-complement wrap the reference's signed loops produce in practice, but defined.
 #include "px_oracle.h"
@@ -457,6 +465,39 @@ void pxo_seg_sort(int dtype, void* data, const uint64_t* offsets, size_t nseg) {
     T* p = static_cast<T*>(data);
     for (size_t s = 0; s < nseg; ++s) std::sort(p + offsets[s], p + offsets[s + 1]);
   })
+}
+
+size_t pxo_find(const px_desc* d, const void* in, size_t n) {
+  PXO_DISPATCH(d->dtype, {
+    const T* p = static_cast<const T*>(in);
+    for (size_t i = 0; i < n; ++i)
+      if (pred<T>(*d, p[i])) return i;
+  })
+  return n;
+}
+
+size_t pxo_mismatch(int dtype, const void* a, const void* b, size_t n) {
+  PXO_DISPATCH(dtype, {
+    const T* x = static_cast<const T*>(a);
+    const T* y = static_cast<const T*>(b);
+    for (size_t i = 0; i < n; ++i)
+      if (!(x[i] == y[i])) return i;
+  })
+  return n;
+}
+
+size_t pxo_argminmax(int dtype, const void* in, size_t n, int want_max, int want_last) {
+  if (n == 0) return 0;
+  size_t best = 0;
+  PXO_DISPATCH(dtype, {
+    const T* p = static_cast<const T*>(in);
+    for (size_t i = 1; i < n; ++i) {
+      const bool strictly = want_max ? (p[best] < p[i]) : (p[i] < p[best]);
+      const bool tie = !(p[i] < p[best]) && !(p[best] < p[i]);
+      if (strictly || (tie && want_last)) best = i;
+    }
+  })
+  return best;
 }
 
 size_t pxo_seg_unique(int dtype, const void* in, void* out, const uint64_t* offsets, size_t nseg,

@@ -58,6 +58,12 @@ size_t pxo_partition(const px_desc* flags, void* data, size_t n);        /* std:
 size_t pxo_stable_partition(const px_desc* flags, void* data, size_t n); /* std::stable_partition */
 size_t pxo_unique(int dtype, void* data, size_t n);                      /* std::unique */
 long pxo_count_if(const px_desc* flags, const void* in, size_t n);       /* stdpar.hpp:290-292 */
+/* Search (the reference launcher's host combine, kernel_launcher.cpp:1078-1098, 1180-1189,
+   1272-1277): first index passing / differing (n if none); min or max index by <, ties to
+   the smaller index (larger with want_last), 0 for n == 0.  NaN inputs: unpinned. */
+size_t pxo_find(const px_desc* pred, const void* in, size_t n);
+size_t pxo_mismatch(int dtype, const void* a, const void* b, size_t n);
+size_t pxo_argminmax(int dtype, const void* in, size_t n, int want_max, int want_last);
 
 /* element-wise skeletons (stdpar.hpp:81, :119) */
 void pxo_map(const px_desc* d, void* data, size_t n, const void* captures, size_t capture_size);

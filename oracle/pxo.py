@@ -51,6 +51,9 @@ def _load():
         ("pxo_stable_partition", sz, [dp, vp, sz]),
         ("pxo_unique", sz, [i, vp, sz]),
         ("pxo_count_if", ctypes.c_long, [dp, vp, sz]),
+        ("pxo_find", sz, [dp, vp, sz]),
+        ("pxo_mismatch", sz, [i, vp, vp, sz]),
+        ("pxo_argminmax", sz, [i, vp, sz, i, i]),
         ("pxo_map", None, [dp, vp, sz, vp, sz]),
         ("pxo_transform", None, [dp, vp, vp, sz, vp, sz]),
         ("pxo_seg_reduce", None, [i, vp, u64p, sz, vp]),
@@ -143,6 +146,24 @@ def unique(a):
 
 def count_if(d, a):
     return lib.pxo_count_if(ctypes.byref(desc(d)), _p(a), a.size)
+
+
+def find(d, a):
+    """First index whose element passes descriptor d's predicate; a.size if none."""
+    a = np.ascontiguousarray(a)
+    return int(lib.pxo_find(desc(d), _p(a), a.size))
+
+
+def mismatch(a, b):
+    """First index with a[i] != b[i]; a.size if the ranges are equal."""
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return int(lib.pxo_mismatch(code(a), _p(a), _p(b), a.size))
+
+
+def argminmax(a, want_max=False, want_last=False):
+    """Index of the minimum (maximum) by <, ties to the first (last) index; 0 if empty."""
+    a = np.ascontiguousarray(a)
+    return int(lib.pxo_argminmax(code(a), _p(a), a.size, int(want_max), int(want_last)))
 
 
 def pred_eval(d, x):

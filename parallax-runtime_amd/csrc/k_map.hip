@@ -10,6 +10,7 @@
 // Search kernels (reference launch_find / launch_argminmax / launch_mismatch,
 // kernel_launcher.cpp:987-1278) finish on the device instead of combining per-block
 // winners on the host.
+#include <algorithm>
 #include <type_traits>
 
 #include "dev_common.hpp"
@@ -90,22 +91,124 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_find(const T* in, uint64_t n, px_desc pd, uint64_t* out) {
-  const Pred<T> pred(pd);
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
-  uint64_t best = n;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-    if (pred(in[i])) {
-      best = i;
+// First index (relative to lo) in [lo, hi) of in_al whose element passes the test.
+// Vectors of 16 bytes from the aligned-down pointer; the head and tail of fewer than VEC
+// elements are tested by block 0.  Blocks run in rounds of UL vectors per thread in
+// increasing index order: a round's hits are reduced in LDS and published with one
+// device-scope min, after which the block stops (its later rounds hold larger indices);
+// a block also stops once a published hit lies below its next round.  The published
+// minimum only decreases, so a stale read of it only delays a stop.
+constexpr int kSearchUL = 4;
+
+template <class T, class Test>
+__device__ __forceinline__ void search_first(int64_t lo, int64_t hi, uint64_t* out, const Test& test) {
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int UL = kSearchUL;
+  __shared__ unsigned long long s_min;
+  __shared__ unsigned long long s_seen;
+  const uint64_t n = static_cast<uint64_t>(hi - lo);
+  const int64_t a = (lo + VEC - 1) / VEC;  // interior vectors [a, b)
+  const int64_t b = hi / VEC;
+  if (blockIdx.x == 0 && threadIdx.x < 2 * VEC) {
+    int64_t pos;
+    bool ok;
+    if (threadIdx.x < VEC) {
+      pos = lo + threadIdx.x;
+      ok = pos < a * VEC && pos < hi;
+    } else {
+      pos = (a > b ? a : b) * VEC + (threadIdx.x - VEC);
+      ok = pos < hi;
+    }
+    if (ok && test.one(pos))
+      atomicMin(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(pos - lo));
+  }
+  if (threadIdx.x == 0) {
+    s_min = n;
+    s_seen = ld_agent(out);
+  }
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  for (int64_t r = a + static_cast<int64_t>(blockIdx.x) * 256; r < b; r += UL * stride) {
+    if (s_seen + static_cast<uint64_t>(lo) <= static_cast<uint64_t>(r * VEC)) break;  // uniform
+    typename Test::Group g;
+#pragma unroll
+    for (int u = 0; u < UL; ++u) {
+      const int64_t v = r + u * stride + threadIdx.x;
+      if (v < b) test.load(g, u, v);
+    }
+    const uint64_t next_seen = threadIdx.x == 0 ? ld_agent(out) : 0ull;  // in flight with the loads
+    uint64_t best = n;
+#pragma unroll
+    for (int u = 0; u < UL; ++u) {
+      const int64_t v = r + u * stride + threadIdx.x;
+      if (v < b && best == n) {
+        const uint32_t m = test.mask(g, u);
+        if (m) best = static_cast<uint64_t>(v * VEC + __builtin_ctz(m) - lo);
+      }
+    }
+    if (best < n) atomicMin(&s_min, static_cast<unsigned long long>(best));
+    __syncthreads();
+    const uint64_t found = s_min;
+    if (found < n) {  // uniform
+      if (threadIdx.x == 0) atomicMin(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(found));
       break;
     }
+    if (threadIdx.x == 0) s_seen = next_seen;
+    __syncthreads();
   }
-  if (best < n) atomicMin(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(best));
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_mismatch(const T* a, const T* b, uint64_t n, uint64_t* out) {
+struct FindTest {
+  using V = Vec<T, 16 / sizeof(T)>;
+  struct Group {
+    V x[kSearchUL];
+  };
+  const V* vin;
+  const T* in_al;
+  Pred<T> pred;
+  __device__ void load(Group& g, int u, int64_t v) const { g.x[u] = ld_stream(vin + v); }
+  __device__ uint32_t mask(const Group& g, int u) const { return pred.template mask<16 / sizeof(T)>(g.x[u].v); }
+  __device__ bool one(int64_t pos) const { return pred(in_al[pos]); }
+};
+
+template <class T>
+struct MismatchTest {
+  static constexpr int VEC = 16 / sizeof(T);
+  using V = Vec<T, VEC>;
+  struct Group {
+    V x[kSearchUL], y[kSearchUL];
+  };
+  const T* a_al;
+  const T* b_al;
+  __device__ void load(Group& g, int u, int64_t v) const {
+    g.x[u] = ld_stream(reinterpret_cast<const V*>(a_al) + v);
+    g.y[u] = ld_stream(reinterpret_cast<const V*>(b_al) + v);
+  }
+  __device__ uint32_t mask(const Group& g, int u) const {
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) m |= (g.x[u].v[j] == g.y[u].v[j] ? 0u : 1u) << j;
+    return m;
+  }
+  __device__ bool one(int64_t pos) const { return !(a_al[pos] == b_al[pos]); }
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_find(const T* in_al, int64_t lo, int64_t hi, px_desc pd, uint64_t* out) {
+  const FindTest<T> t{reinterpret_cast<const typename FindTest<T>::V*>(in_al), in_al, Pred<T>(pd)};
+  search_first<T>(lo, hi, out, t);
+}
+
+// a_al and b_al share their offset from a 16-byte boundary (else k_mismatch_scalar).
+template <class T>
+__global__ __launch_bounds__(256) void k_mismatch(const T* a_al, const T* b_al, int64_t lo, int64_t hi, uint64_t* out) {
+  const MismatchTest<T> t{a_al, b_al};
+  search_first<T>(lo, hi, out, t);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_mismatch_scalar(const T* a, const T* b, uint64_t n, uint64_t* out) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
     if (!(a[i] == b[i])) {
@@ -133,17 +236,61 @@ __device__ __forceinline__ bool better(const Best<T>& a, const Best<T>& b, bool 
   return want_last ? (a.i > b.i) : (a.i < b.i);
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_argminmax(const T* in, uint64_t n, bool want_max,
-                                                   bool want_last, Best<T>* partials, Ctl* ctl,
-                                                   uint64_t* out) {
+// A thread visits its elements in increasing index order, so one comparison per element
+// keeps its winner (strict for the first index among ties, non-strict for the last);
+// winners of different threads combine by better().
+template <class T, bool MAX, bool LAST>
+__global__ __launch_bounds__(256) void k_argminmax(const T* in_al, int64_t lo, int64_t hi,
+                                                   Best<T>* partials, Ctl* ctl, uint64_t* out) {
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int UL = 8;  // vector loads in flight per thread
+  constexpr bool want_max = MAX, want_last = LAST;
+  using V = Vec<T, VEC>;
   __shared__ Best<T> s_b[256];
   __shared__ int s_last;
+  const uint64_t n = static_cast<uint64_t>(hi - lo);
   Best<T> b{T{}, n};
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-    const Best<T> c{in[i], i};
-    if (better(c, b, want_max, want_last, n)) b = c;
+  bool have = false;
+  auto take = [&](T x, int64_t pos) {
+    bool win;
+    if constexpr (MAX)
+      win = LAST ? !(x < b.v) : (b.v < x);
+    else
+      win = LAST ? !(b.v < x) : (x < b.v);
+    if (!have || win) {
+      b.v = x;
+      b.i = static_cast<uint64_t>(pos - lo);
+      have = true;
+    }
+  };
+  const int64_t va = (lo + VEC - 1) / VEC;  // interior vectors [va, vb)
+  const int64_t vb = hi / VEC;
+  // the head (before the first whole vector) first and the tail last, so every thread
+  // still sees its elements in increasing index order
+  if (blockIdx.x == 0 && threadIdx.x < VEC) {
+    const int64_t pos = lo + threadIdx.x;
+    if (pos < va * VEC && pos < hi) take(in_al[pos], pos);
+  }
+  const V* vin = reinterpret_cast<const V*>(in_al);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  int64_t v = va + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  for (; v + (UL - 1) * stride < vb; v += UL * stride) {
+    V x[UL];
+#pragma unroll
+    for (int u = 0; u < UL; ++u) x[u] = ld_stream(vin + v + u * stride);
+#pragma unroll
+    for (int u = 0; u < UL; ++u)
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) take(x[u].v[j], (v + u * stride) * VEC + j);
+  }
+  for (; v < vb; v += stride) {
+    const V x = ld_stream(vin + v);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) take(x.v[j], v * VEC + j);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < VEC) {
+    const int64_t pos = (va > vb ? va : vb) * VEC + threadIdx.x;
+    if (pos < hi) take(in_al[pos], pos);
   }
   s_b[threadIdx.x] = b;
   __syncthreads();
@@ -237,14 +384,34 @@ hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Aligned-down pointer and element range [lo, lo + n) of an operand.
+template <class T>
+inline const T* aligned_down(const void* p, int64_t& lo) {
+  const uintptr_t u = reinterpret_cast<uintptr_t>(p);
+  lo = static_cast<int64_t>((u & 15u) / sizeof(T));
+  return reinterpret_cast<const T*>(u & ~static_cast<uintptr_t>(15));
+}
+
+// Search grids: at most per_cu 256-thread blocks per CU (measured at 2^28 i32, 4 / 8 per
+// CU: find with no hit 215 / 161 us, mismatch 323 / 361 us), no more blocks than rounds
+// of kSearchUL vectors need.
+inline uint32_t search_grid(uint64_t n, size_t elem, const Scratch& s, uint32_t per_cu) {
+  const uint64_t per_block = 256ull * kSearchUL * (16 / elem);
+  const uint64_t cap = std::max<uint64_t>(static_cast<uint64_t>(s.max_blocks / 8) * per_cu, 1);
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<uint32_t>(g);
+}
+
 hipError_t find_first(const px_desc& pred, const void* in, uint64_t n, uint64_t* out_dev,
                       const Scratch& s, hipStream_t stream) {
-  (void)s;
   k_set_u64<<<1, 1, 0, stream>>>(out_dev, n);
   if (n == 0) return hipGetLastError();
-  const uint32_t grid = ew_grid(n, s.max_blocks);
   PXK_DISPATCH(pred.dtype, {
-    k_find<T><<<grid, 256, 0, stream>>>(reinterpret_cast<const T*>(in), n, pred, out_dev);
+    int64_t lo;
+    const T* al = aligned_down<T>(in, lo);
+    k_find<T><<<search_grid(n, sizeof(T), s, 8), 256, 0, stream>>>(al, lo, lo + static_cast<int64_t>(n), pred, out_dev);
     return hipGetLastError();
   });
   return hipErrorInvalidValue;
@@ -254,10 +421,17 @@ hipError_t mismatch(int dtype, const void* a, const void* b, uint64_t n, uint64_
                     const Scratch& s, hipStream_t stream) {
   k_set_u64<<<1, 1, 0, stream>>>(out_dev, n);
   if (n == 0) return hipGetLastError();
-  const uint32_t grid = ew_grid(n, s.max_blocks);
   PXK_DISPATCH(dtype, {
-    k_mismatch<T><<<grid, 256, 0, stream>>>(reinterpret_cast<const T*>(a),
-                                            reinterpret_cast<const T*>(b), n, out_dev);
+    int64_t lo, lo_b;
+    const T* aa = aligned_down<T>(a, lo);
+    const T* ba = aligned_down<T>(b, lo_b);
+    if (lo == lo_b && (reinterpret_cast<uintptr_t>(a) & (sizeof(T) - 1)) == 0 &&
+        (reinterpret_cast<uintptr_t>(b) & (sizeof(T) - 1)) == 0) {
+      k_mismatch<T><<<search_grid(n, sizeof(T), s, 4), 256, 0, stream>>>(aa, ba, lo, lo + static_cast<int64_t>(n), out_dev);
+    } else {  // different offsets from a 16-byte boundary: element loads
+      k_mismatch_scalar<T><<<ew_grid(n, s.max_blocks), 256, 0, stream>>>(reinterpret_cast<const T*>(a),
+                                                                         reinterpret_cast<const T*>(b), n, out_dev);
+    }
     return hipGetLastError();
   });
   return hipErrorInvalidValue;
@@ -269,11 +443,25 @@ hipError_t argminmax(int dtype, const void* in, uint64_t n, bool want_max, bool 
     k_set_u64<<<1, 1, 0, stream>>>(out_dev, 0);
     return hipGetLastError();
   }
-  const uint32_t grid = ew_grid(n, s.max_blocks);
   PXK_DISPATCH(dtype, {
+    // four 256-thread blocks per CU at most (measured: 2 and 8 per CU slower), 8 vectors
+    // per thread per round
+    const uint64_t per_block = 256ull * 8 * (16 / sizeof(T));
+    const uint64_t cap = s.max_blocks / 2 ? s.max_blocks / 2 : 1;
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>((n + per_block - 1) / per_block, 1), cap));
     if (grid * sizeof(Best<T>) > s.partial_bytes) return hipErrorInvalidValue;
-    k_argminmax<T><<<grid, 256, 0, stream>>>(reinterpret_cast<const T*>(in), n, want_max, want_last,
-                                             reinterpret_cast<Best<T>*>(s.partials), s.ctl, out_dev);
+    int64_t lo;
+    const T* al = aligned_down<T>(in, lo);
+    Best<T>* part = reinterpret_cast<Best<T>*>(s.partials);
+    const int64_t hi = lo + static_cast<int64_t>(n);
+    if (want_max && want_last)
+      k_argminmax<T, true, true><<<grid, 256, 0, stream>>>(al, lo, hi, part, s.ctl, out_dev);
+    else if (want_max)
+      k_argminmax<T, true, false><<<grid, 256, 0, stream>>>(al, lo, hi, part, s.ctl, out_dev);
+    else if (want_last)
+      k_argminmax<T, false, true><<<grid, 256, 0, stream>>>(al, lo, hi, part, s.ctl, out_dev);
+    else
+      k_argminmax<T, false, false><<<grid, 256, 0, stream>>>(al, lo, hi, part, s.ctl, out_dev);
     return hipGetLastError();
   });
   return hipErrorInvalidValue;

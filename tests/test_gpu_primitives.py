@@ -401,3 +401,58 @@ def test_search_entry_points(gpu_ctx):
     check(lib.pxd_mismatch(gpu_ctx.h, PX_I32, ip, bp, a.size, idx.ptr), "mismatch")
     gpu_ctx.sync()
     assert idx.download(np.uint64, 1)[0] == 99_000
+
+
+SEARCH_SIZES = [1, 3, 17, 1000, 4099, 65_537, 1_000_003, 3_000_017]
+
+
+def _search(gpu_ctx, fn, *args):
+    idx = DeviceBuffer(8)
+    check(fn(gpu_ctx.h, *args, idx.ptr), "search")
+    gpu_ctx.sync()
+    return int(idx.download(np.uint64, 1)[0])
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", SEARCH_SIZES)
+def test_search_against_oracle(gpu_ctx, dt, n):
+    """find / argminmax / mismatch against the oracle: hits at the start, the end, inside
+    and nowhere; ties both ways; pointers off a 16-byte boundary (mismatch with equal
+    and with different offsets)."""
+    rng = np.random.default_rng(990 + n % 61 + dt)
+    t = NP_DTYPES[dt]
+    a = rng.integers(0, 1000, n).astype(t)  # ties for argminmax
+    big = make_desc(C.OP_FIND, dt, fn=C.PRED_GT, c0=5000)
+    for shift in (0, 1, 3):
+        for pos in sorted({0, n - 1, n // 2, (7 * n) // 9}):
+            x = a.copy()
+            x[pos] = t.type(7000)
+            if pos + 5 < n:
+                x[pos + 5] = t.type(7000)  # a later hit and a tie for the maximum
+            b, p = upload(x, shift)
+            got = _search(gpu_ctx, lib.pxd_find, ctypes.byref(big), p, n)
+            assert got == pxo.find(big, x) == pos, (shift, pos)
+        b, p = upload(a, shift)
+        assert _search(gpu_ctx, lib.pxd_find, ctypes.byref(big), p, n) == pxo.find(big, a) == n
+        for want_max in (0, 1):
+            for want_last in (0, 1):
+                got = _search(gpu_ctx, lib.pxd_argminmax, dt, p, n, want_max, want_last)
+                assert got == pxo.argminmax(a, want_max, want_last), (shift, want_max, want_last)
+        for shift_b in (shift, (shift + 1) % 4):
+            y = a.copy()
+            bb, pb = upload(y, shift_b)
+            assert _search(gpu_ctx, lib.pxd_mismatch, dt, p, pb, n) == n
+            k = int(rng.integers(0, n))
+            y[k] += y.dtype.type(1)
+            if k + 9 < n:
+                y[k + 9] += y.dtype.type(1)
+            bb, pb = upload(y, shift_b)
+            assert _search(gpu_ctx, lib.pxd_mismatch, dt, p, pb, n) == pxo.mismatch(a, y) == k, (shift, shift_b)
+
+
+def test_search_empty(gpu_ctx):
+    b, p = upload(np.zeros(4, np.float32))
+    d = make_desc(C.OP_FIND, PX_F32, fn=C.PRED_GT, c0=0.0)
+    assert _search(gpu_ctx, lib.pxd_find, ctypes.byref(d), p, 0) == 0
+    assert _search(gpu_ctx, lib.pxd_mismatch, PX_F32, p, p, 0) == 0
+    assert _search(gpu_ctx, lib.pxd_argminmax, PX_F32, p, 0, 0, 0) == 0

@@ -211,3 +211,55 @@ def test_segmented_equal_per_segment_plain():
         assert np.array_equal(s[off[i]:off[i + 1]], pxo.inclusive_scan(seg))
         assert np.array_equal(so[off[i]:off[i + 1]], np.sort(seg))
         assert np.array_equal(uq[uo[i]:uo[i + 1]], pxo.unique(np.sort(seg)))
+
+
+# ---- search (find / mismatch / argminmax): numpy routines and brute force
+@pytest.mark.parametrize("dt", ALL)
+def test_argminmax_matches_numpy_and_brute_force(dt):
+    rng = np.random.default_rng(40 + dt)
+    t = NP_DTYPES[dt]
+    for n in (1, 2, 7, 100, 1001):
+        a = rng.integers(0, 20, n).astype(t)  # many ties
+        assert pxo.argminmax(a) == int(np.argmin(a))
+        assert pxo.argminmax(a, want_max=True) == int(np.argmax(a))
+        assert pxo.argminmax(a, want_last=True) == n - 1 - int(np.argmin(a[::-1]))
+        assert pxo.argminmax(a, want_max=True, want_last=True) == n - 1 - int(np.argmax(a[::-1]))
+        if n <= 100:  # pure-Python loops over the values
+            v = a.tolist()
+            lo, hi = min(v), max(v)
+            assert pxo.argminmax(a) == v.index(lo)
+            assert pxo.argminmax(a, want_max=True, want_last=True) == max(i for i, x in enumerate(v) if x == hi)
+    assert pxo.argminmax(np.zeros(0, t)) == 0
+    assert pxo.argminmax(np.full(9, 3, t), want_last=True) == 8 and pxo.argminmax(np.full(9, 3, t)) == 0
+
+
+def test_argminmax_signed_zero_ties_and_sign():
+    a = np.array([0.0, -0.0, -1.0, 5.0, -1.0], np.float64)
+    assert pxo.argminmax(a) == 2 and pxo.argminmax(a, want_last=True) == 4
+    z = np.array([0.0, -0.0], np.float32)  # -0.0 == 0.0: a tie
+    assert pxo.argminmax(z) == 0 and pxo.argminmax(z, want_last=True) == 1
+    i = np.array([-(1 << 62), 1 << 62, -(1 << 62) + 1], np.int64)  # exact beyond 2^53
+    assert pxo.argminmax(i) == 0 and pxo.argminmax(i, want_max=True) == 1
+
+
+@pytest.mark.parametrize("dt", ALL)
+def test_find_and_mismatch_match_numpy(dt):
+    rng = np.random.default_rng(50 + dt)
+    t = NP_DTYPES[dt]
+    for n in (0, 1, 5, 300, 4099):
+        a = rng.integers(0, 100, n).astype(t)
+        d = make_desc(C.OP_FIND, dt, fn=C.PRED_GT, c0=90)
+        hits = np.flatnonzero(a > 90)
+        assert pxo.find(d, a) == (int(hits[0]) if hits.size else n)
+        dn = make_desc(C.OP_FIND, dt, fn=C.PRED_GT, c0=1000)  # never true
+        assert pxo.find(dn, a) == n
+        b = a.copy()
+        assert pxo.mismatch(a, b) == n
+        if n:
+            k = int(rng.integers(0, n))
+            one = b.dtype.type(1)
+            b[k] += one
+            b[min(n - 1, k + 3)] += one
+            assert pxo.mismatch(a, b) == int(np.flatnonzero(a != b)[0]) == k
+    z = np.array([0.0, 1.0], np.float64)
+    assert pxo.mismatch(z, np.array([-0.0, 1.0])) == 2  # -0.0 == 0.0
