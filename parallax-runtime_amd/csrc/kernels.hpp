@@ -181,9 +181,9 @@ uint64_t radix_status_words(int dtype, uint64_t n);
 int radix_passes(int dtype);
 hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipStream_t stream,
                       const LaunchHook& hook = {});
-// Segmented radix sort of the long segments: an OR pass (digits every key shares are
-// skipped on the device), per-segment digit counts, then one pass per digit that runs,
-// the last writing the array.  Scratch: 256 status words per tile; partials hold
+// Segmented radix sort of the long segments: one counting pass (per-segment digit counts,
+// and the digits every key shares, whose passes are skipped on the device), then one pass
+// per digit that runs, the last writing the array.  Scratch: 256 status words per tile; partials hold
 // the second and third copies (indices up to the largest segment end; the third is
 // seg.tmp2) and seg.seg_cnt (both set by the caller inside partials,
 // seg_radix_cnt_bytes()).

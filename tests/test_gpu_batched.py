@@ -134,8 +134,8 @@ def test_seg_sort_radix(gpu_ctx, dt, kind):
 
 @pytest.mark.parametrize("dt", ALL)
 def test_seg_sort_radix_unaligned(gpu_ctx, dt):
-    """Segmented radix with the array one element past a 16-byte boundary (the OR pass
-    reads each tile's unaligned head and tail one key at a time)."""
+    """Segmented radix with the array one element past a 16-byte boundary (the counting
+    pass reads each tile's unaligned head and tail one key at a time)."""
     a, off = radix_seg_input(dt, 80 + dt, "full")
     item = a.itemsize
     d = DeviceBuffer(a.nbytes + 64)
