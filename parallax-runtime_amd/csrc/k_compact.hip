@@ -69,8 +69,129 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
 
   // Positions inside the tile window are handled as 32-bit offsets rel in [0, TILE):
   // thread element e = k*VEC + j sits at rel = base_rel(k) + j; the segment covers
-  // [lo_r, hi_r) of the window (lo_r = -1 when the segment starts before it).
+  // [lo_r, hi_r) of the window (lo_r = -1 when the segment starts before it; an empty
+  // range when it ends before the window or starts after it).
   auto base_rel = [&](int k) { return ((wid * K + k) * kWave + lane) * VEC; };
+
+  struct Tile {
+    T x[E];
+    uint32_t flags, valid;
+    C tex[K];
+    C wave_off, kept;
+    int lo_r, hi_r, vstart_r;
+    int64_t w0;
+  };
+
+  // Loads, flags and ranks the tile (ends with the block's counts in s_w read out).
+  auto load_rank = [&](Tile& u, const TileInfo& ti) {
+    u.w0 = ti.w0;
+    const int64_t dl = ti.lo - u.w0, dh = ti.hi - u.w0;
+    u.lo_r = dl < 0 ? -1 : (dl > TILE ? TILE : static_cast<int>(dl));
+    u.hi_r = dh > TILE ? TILE : (dh < 0 ? 0 : static_cast<int>(dh));
+    u.vstart_r = u.lo_r > 0 ? u.lo_r : 0;
+    const bool full = u.lo_r <= 0 && u.hi_r == TILE;
+    const V* tp = vin + u.w0 / VEC;
+
+    // ---- raw loads (chunks wholly outside the segment are not read) and valid bits
+    u.valid = 0;
+    if (full) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const V v = ld_stream(tp + base_rel(k) / VEC);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = v.v[j];
+      }
+      u.valid = (E == 32) ? 0xffffffffu : ((1u << E) - 1u);
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r0 = base_rel(k);
+        if (r0 < u.hi_r && r0 + VEC > u.lo_r) {
+          const V v = tp[r0 / VEC];
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = v.v[j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          if (r0 + j >= u.lo_r && r0 + j < u.hi_r) u.valid |= 1u << (k * VEC + j);
+      }
+    }
+
+    // ---- flags: the predicate (or adjacent difference) on valid positions
+    if constexpr (UNIQUE) {
+      u.flags = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r0 = base_rel(k);
+        // previous element of lane's first item: lane-1's last item, or memory for lane 0
+        T prev = dpp_mov<kWaveShr1, 0xf, 0xf>(u.x[k * VEC + VEC - 1]);
+        if (lane == 0 && r0 - 1 >= u.lo_r && r0 - 1 < u.hi_r) prev = in_al[u.w0 + r0 - 1];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const T pv = (j == 0) ? prev : u.x[k * VEC + j - 1];
+          if ((r0 + j == u.lo_r) || !(u.x[k * VEC + j] == pv)) u.flags |= 1u << (k * VEC + j);
+        }
+      }
+      u.flags &= u.valid;
+    } else {
+      u.flags = pred.template mask<E>(u.x) & u.valid;
+    }
+
+    // ---- rank kept elements: per-chunk wave scans of counts
+    C run = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const C c = static_cast<C>(__builtin_popcount((u.flags >> (k * VEC)) & ((1u << VEC) - 1u)));
+      const C incl = wave_incl_scan(c);
+      u.tex[k] = run + wave_shift_right1(incl);
+      run += read_lane(incl, kWave - 1);
+    }
+    if (lane == 0) s_w[wid] = run;
+    __syncthreads();
+    u.wave_off = 0;
+    for (int j = 0; j < wid; ++j) u.wave_off += s_w[j];
+    u.kept = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) u.kept += s_w[j];
+  };
+
+  // Stages the kept (and, for partition, rejected) elements in LDS, in order.
+  auto stage = [&](const Tile& u) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      C r = u.wave_off + u.tex[k];
+      const int r0 = base_rel(k);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const int e = k * VEC + j;
+        if ((u.flags >> e) & 1u) {
+          s_stage[r] = u.x[e];
+          ++r;
+        } else if (MODE == PX_SCATTER_PARTITION) {
+          if ((u.valid >> e) & 1u) {
+            const C vi = static_cast<C>(r0 + j - u.vstart_r);  // valid index inside the tile
+            s_stage[u.kept + (vi - r)] = u.x[e];
+          }
+        }
+      }
+    }
+  };
+
+  // Writes the staged run(s); kept_before = kept in the segment before the tile.
+  auto write = [&](const Tile& u, const TileInfo& ti, uint64_t seg_out, uint64_t kept_before) {
+    write_run<T, BLOCK>(out + seg_out + kept_before, s_stage, u.kept);
+    if (MODE == PX_SCATTER_PARTITION) {
+      const uint64_t num_true = *num_true_dev;
+      const C sub_valid = static_cast<C>(u.hi_r > u.vstart_r ? u.hi_r - u.vstart_r : 0);
+      if (sub_valid > 0) {
+        const uint64_t rej_before = static_cast<uint64_t>(u.w0 + u.vstart_r - ti.lo) - kept_before;
+        write_run<T, BLOCK>(out + seg_out + num_true + rej_before, s_stage + u.kept, sub_valid - u.kept);
+      }
+    }
+  };
 
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
 
@@ -84,102 +205,13 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     const uint32_t epoch = s_epoch;
     if (tile >= g.ntiles) break;
     const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
-    const int lo_r = ti.lo - ti.w0 < 0 ? -1 : static_cast<int>(ti.lo - ti.w0);
-    const int hi_r = ti.hi - ti.w0 > TILE ? TILE : static_cast<int>(ti.hi - ti.w0);
-    const bool full = lo_r <= 0 && hi_r == TILE;
-    const V* tp = vin + ti.w0 / VEC;
 
-    // ---- raw loads (chunks wholly outside the segment are not read) and valid bits
-    T x[E];
-    uint32_t valid = 0;
-    if (full) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const V v = ld_stream(tp + base_rel(k) / VEC);
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) x[k * VEC + j] = v.v[j];
-      }
-      valid = (E == 32) ? 0xffffffffu : ((1u << E) - 1u);
-    } else {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int r0 = base_rel(k);
-        if (r0 < hi_r && r0 + VEC > lo_r) {
-          const V v = tp[r0 / VEC];
-#pragma unroll
-          for (int j = 0; j < VEC; ++j) x[k * VEC + j] = v.v[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < VEC; ++j) x[k * VEC + j] = T(0);
-        }
-#pragma unroll
-        for (int j = 0; j < VEC; ++j)
-          if (r0 + j >= lo_r && r0 + j < hi_r) valid |= 1u << (k * VEC + j);
-      }
-    }
-
-    // ---- flags: the predicate (or adjacent difference) on valid positions
-    uint32_t flags;
-    if constexpr (UNIQUE) {
-      flags = 0;
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int r0 = base_rel(k);
-        // previous element of lane's first item: lane-1's last item, or memory for lane 0
-        T prev = dpp_mov<kWaveShr1, 0xf, 0xf>(x[k * VEC + VEC - 1]);
-        if (lane == 0 && r0 - 1 >= lo_r && r0 - 1 < hi_r) prev = in_al[ti.w0 + r0 - 1];
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-          const T pv = (j == 0) ? prev : x[k * VEC + j - 1];
-          if ((r0 + j == lo_r) || !(x[k * VEC + j] == pv)) flags |= 1u << (k * VEC + j);
-        }
-      }
-      flags &= valid;
-    } else {
-      flags = pred.template mask<E>(x) & valid;
-    }
-
-    // ---- rank kept elements: per-chunk wave scans of counts
-    C run = 0;
-    C tex[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const C c = static_cast<C>(__builtin_popcount((flags >> (k * VEC)) & ((1u << VEC) - 1u)));
-      const C incl = wave_incl_scan(c);
-      tex[k] = run + wave_shift_right1(incl);
-      run += read_lane(incl, kWave - 1);
-    }
-    if (lane == 0) s_w[wid] = run;
-    __syncthreads();
-    C wave_off = 0;
-    for (int j = 0; j < wid; ++j) wave_off += s_w[j];
-    C tile_kept = 0;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) tile_kept += s_w[j];
+    Tile a;
+    load_rank(a, ti);
+    const C tile_kept = a.kept;
     // publish the tile's count before staging, so later tiles' look-backs need not wait
     if (threadIdx.x == 0 && tile != ti.first_tile) publish<C>(status, tile, epoch, kStAgg, tile_kept);
-    const int vstart_r = lo_r > 0 ? lo_r : 0;  // first valid offset of the window
-    const C tile_valid = static_cast<C>(hi_r - vstart_r);
-
-    // ---- stage kept (and, for partition, rejected) elements in LDS, in order
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      C r = wave_off + tex[k];
-      const int r0 = base_rel(k);
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const int e = k * VEC + j;
-        if ((flags >> e) & 1u) {
-          s_stage[r] = x[e];
-          ++r;
-        } else if (MODE == PX_SCATTER_PARTITION) {
-          if ((valid >> e) & 1u) {
-            const C vi = static_cast<C>(r0 + j - vstart_r);  // valid index inside the tile
-            s_stage[tile_kept + (vi - r)] = x[e];
-          }
-        }
-      }
-    }
+    stage(a);
 
     // ---- kept before this tile (look-back), then the tile's count
     if (wid == 0) {
@@ -205,13 +237,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     // ---- coalesced writes of the staged run(s)
     const uint64_t P = s_prefix;
     const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
-    write_run<T, BLOCK>(out + seg_out + P, s_stage, tile_kept);
-    if (MODE == PX_SCATTER_PARTITION) {
-      const uint64_t num_true = *num_true_dev;
-      const uint64_t rej_before = static_cast<uint64_t>(ti.w0 + vstart_r - ti.lo) - P;
-      write_run<T, BLOCK>(out + seg_out + num_true + rej_before, s_stage + tile_kept,
-                          tile_valid - tile_kept);
-    }
+    write(a, ti, seg_out, P);
     __syncthreads();
   }
   if (threadIdx.x == 0) block_end(ctl, s_epoch);

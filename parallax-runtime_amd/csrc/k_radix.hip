@@ -359,7 +359,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (threadIdx.x == 0)
       s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-    for (int b = lane; b < kBins; b += kWave) s_wh[wid][b] = 0u;
+    for (int q = 0; q < kBins / kWave; ++q) s_wh[wid][q * kWave + lane] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t epoch = s_epoch;
