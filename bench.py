@@ -232,6 +232,8 @@ def plain_suite(args):
     assert int(cres[0]) == even, ("count_if", int(cres[0]), even)
     fused["parity"] = {"square_sum_err_over_bound": abs(fres[0] - exact) / bnd, "count_if": "exact"}
     log(f"[bench] fused map-reduce: {fused}")
+    search = search_suite(args, X, D, n28)
+    log(f"[bench] search: {search}")
     ach = kernels[dom]["gbps"]
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
@@ -246,8 +248,52 @@ def plain_suite(args):
                                "sort 2^26 u32 + 2^25 f64, copy_if 2^28 i32 (x%2==0), seed 1",
                    "scale": scale},
         "roofline": roof, "per_primitive": per, "kernels": kernels, "parity": parity,
-        "c_abi_host_operands": host, "fused_map_reduce": fused,
+        "c_abi_host_operands": host, "fused_map_reduce": fused, "search": search,
     }
+
+
+def search_suite(args, X, D, n):
+    """find / argminmax / mismatch over the suite's 2^28 inputs (device-resident, one
+    stream; device time per call from events, median of --steps calls): find with a
+    predicate no element passes (the whole array is read), argmin of the f32 input,
+    argmax of the i32 input, mismatch of the i32 input with a copy differing 5 elements
+    from the end.  Algorithmic bytes: n*s (mismatch 2*n*s)."""
+    import ctypes
+    import statistics
+    from parallax_amd import lib, check, make_desc, consts as C
+    from parallax_amd.device import Context, DeviceBuffer
+    from parallax_amd.dtypes import PX_F32, PX_I32
+    ctx = Context()
+    idx = DeviceBuffer(8)
+    other = DeviceBuffer.from_array(X["copy_if_i32"])
+    k = n - 5
+    bumped = np.array([X["copy_if_i32"][k] ^ 1], dtype=np.int32)
+    check(lib.pxd_memcpy_h2d(other.addr(4 * k), bumped.ctypes.data, 4), "upload")
+    never = make_desc(C.OP_FIND, PX_I32, fn=C.PRED_GT, c0=(1 << 31) - 1)
+    cases = [
+        ("find_i32_no_hit", n * 4, n, lambda: lib.pxd_find(ctx.h, ctypes.byref(never), D["cif_in"].ptr, n, idx.ptr)),
+        ("argmin_f32", n * 4, int(np.argmin(X["reduce_f32"])),
+         lambda: lib.pxd_argminmax(ctx.h, PX_F32, D["red_f32"].ptr, n, 0, 0, idx.ptr)),
+        ("argmax_i32", n * 4, int(np.argmax(X["copy_if_i32"])),
+         lambda: lib.pxd_argminmax(ctx.h, PX_I32, D["cif_in"].ptr, n, 1, 0, idx.ptr)),
+        ("mismatch_i32", 2 * n * 4, k, lambda: lib.pxd_mismatch(ctx.h, PX_I32, D["cif_in"].ptr, other.ptr, n, idx.ptr)),
+    ]
+    out = {}
+    for name, nb, expect, fn in cases:
+        ms = []
+        for i in range(args.warmup + args.steps):
+            ctx.timer_begin()
+            check(fn(), name)
+            t = ctx.timer_end()
+            if i >= args.warmup:
+                ms.append(t)
+        ctx.sync()
+        got = int(idx.download(np.uint64, 1)[0])
+        assert got == expect, (name, got, expect)
+        m = statistics.median(ms)
+        out[name] = {"ms": round(m, 4), "alg_bytes": nb, "gbps": round(nb / (m * 1e-3) / 1e9, 1),
+                     "pct_hbm_peak": round(100 * nb / (m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 1), "index": got}
+    return out
 
 
 def measured_traffic(kernel):
@@ -746,7 +792,7 @@ def main():
             "dtype": r["dtype"], "data": "synthetic", "config": r["config"],
             "roofline": r.get("roofline"), "cpu_baseline": cpu}
     for k in ("per_primitive", "kernels", "autotuned_tiles", "parity", "c_abi_host_operands",
-              "fused_map_reduce", "latency_ms", "req_per_s", "speedup_vs_plain"):
+              "fused_map_reduce", "search", "latency_ms", "req_per_s", "speedup_vs_plain"):
         if k in r:
             line[k] = r[k]
     print(json.dumps(line), flush=True)
