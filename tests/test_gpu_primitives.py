@@ -6,6 +6,7 @@ must satisfy |err| <= 2*ceil(log2 n)*eps*sum|x| against the double-double oracle
 machine epsilon of the element type; for scans sum|x| is taken over each prefix)."""
 import ctypes
 import math
+import os
 
 import numpy as np
 import pytest
@@ -258,6 +259,8 @@ def test_sort_radix_shared_digits(gpu_ctx, dt, kind):
 def test_sort_radix_signed_zero_nan(gpu_ctx):
     """-0.0 / +0.0 are equal under operator<, NaNs unordered: the radix order puts -0.0
     before +0.0 and NaNs by sign at the ends; everything else matches std::sort."""
+    if "sort_radix_min" in os.environ.get("PARALLAX_TUNE", ""):
+        pytest.skip("the radix threshold is overridden (this order is the radix path's)")
     n = (1 << 20) + 1
     rng = np.random.default_rng(5)
     a = rng.standard_normal(n).astype(np.float32)
