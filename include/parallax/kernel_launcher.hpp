@@ -91,6 +91,7 @@ class KernelLauncher {
   struct Operand;
   bool resolve(void* p, size_t bytes, bool read, Operand& op);
   bool finish(Operand& op, bool write);
+  bool copy_out(void* dst, const void* dev_src, size_t bytes);
   void release(Operand& op);
   const px_desc* find(const std::string& name, const char* who) const;
   void prof_begin();

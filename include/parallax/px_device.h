@@ -59,7 +59,8 @@ int pxr_builtin_descriptor(const char* key, unsigned* words12);
 int pxr_prof_enable(int on);
 int pxr_prof_get(const char* key, double* total_ms, uint64_t* calls);
 
-/* ---- device memory helpers (for bindings without a GPU framework) */
+/* ---- device memory helpers (for bindings without a GPU framework); each has completed
+   when it returns, so work queued afterwards on any stream sees its result */
 void* pxd_malloc(size_t bytes);
 void pxd_free(void* p);
 int pxd_memcpy_h2d(void* dst, const void* src, size_t bytes);

@@ -111,21 +111,29 @@ void pxd_free(void* p) {
   if (p) (void)hipFree(p);
 }
 
-int pxd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+// The copy has completed when these return (hipMemcpy alone may return before a
+// device-side or staged copy lands, and work on non-blocking streams is not ordered after
+// the null stream).
+static int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, const char* what) {
   if (!parallax::global_backend()) return nodev();
-  return hip_rc(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "memcpy h2d");
+  const int rc = hip_rc(hipMemcpy(dst, src, bytes, kind), what);
+  if (rc != PX_EOK) return rc;
+  return hip_rc(hipStreamSynchronize(nullptr), what);
+}
+int pxd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return copy_sync(dst, src, bytes, hipMemcpyHostToDevice, "memcpy h2d");
 }
 int pxd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
-  if (!parallax::global_backend()) return nodev();
-  return hip_rc(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "memcpy d2h");
+  return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost, "memcpy d2h");
 }
 int pxd_memcpy_d2d(void* dst, const void* src, size_t bytes) {
-  if (!parallax::global_backend()) return nodev();
-  return hip_rc(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "memcpy d2d");
+  return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice, "memcpy d2d");
 }
 int pxd_memset(void* dst, int value, size_t bytes) {
   if (!parallax::global_backend()) return nodev();
-  return hip_rc(hipMemset(dst, value, bytes), "memset");
+  const int rc = hip_rc(hipMemset(dst, value, bytes), "memset");
+  if (rc != PX_EOK) return rc;
+  return hip_rc(hipStreamSynchronize(nullptr), "memset");
 }
 int pxd_device_sync(void) {
   if (!parallax::global_backend()) return nodev();

@@ -226,7 +226,12 @@ bool KernelLauncher::resolve(void* p, size_t bytes, bool read, Operand& op) {
     op.dev = nullptr;
     return false;
   }
-  if (read && hipMemcpy(op.dev, p, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
+  // the copy has landed before the kernel that reads it is queued on the launcher's
+  // (non-blocking) stream: hipMemcpy from pageable memory may return once the data is
+  // staged, so the null stream is waited for too
+  if (read && (hipMemcpy(op.dev, p, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+               hipStreamSynchronize(nullptr) != hipSuccess))
+    return false;
   return true;
 }
 
@@ -234,9 +239,19 @@ bool KernelLauncher::finish(Operand& op, bool write) {
   if (wait_stream(ctx_->stream()) != hipSuccess) return false;
   if (!write || op.bytes == 0) return true;
   if (op.kind == Operand::ARENA) return arena_->invalidate_range(op.user, op.bytes);
-  if (op.kind == Operand::STAGED)
-    return hipMemcpy(op.user, op.dev, op.bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  if (op.kind == Operand::STAGED) return copy_out(op.user, op.dev, op.bytes);
   return true;
+}
+
+// A result word (or staged operand) to the caller's pointer once the launcher's stream has
+// drained.  To host memory hipMemcpy returns when the copy is done; to device memory it
+// may return before the copy runs, and a copy on the null stream is not ordered before
+// the next call's kernels (which reuse the same device word), so that copy goes on the
+// launcher's stream and is waited for.
+bool KernelLauncher::copy_out(void* dst, const void* dev_src, size_t bytes) {
+  if (!is_device_pointer(dst)) return hipMemcpy(dst, dev_src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  if (hipMemcpyAsync(dst, dev_src, bytes, hipMemcpyDeviceToDevice, ctx_->stream()) != hipSuccess) return false;
+  return wait_stream(ctx_->stream()) == hipSuccess;
 }
 
 void KernelLauncher::release(Operand& op) {
@@ -442,10 +457,7 @@ bool KernelLauncher::launch_reduce(const std::string& kernel_name, void* data, s
   }
   ok = finish(op, false) && ok;
   release(op);
-  if (ok) {
-    const hipMemcpyKind kind = is_device_pointer(out_result) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    ok = hipMemcpy(out_result, dev_value_, elem_size, kind) == hipSuccess;
-  }
+  if (ok) ok = copy_out(out_result, dev_value_, elem_size);
   if (!ok) report("launch_reduce", last_error_text());
   return ok;
 }
@@ -490,10 +502,7 @@ bool KernelLauncher::launch_transform_reduce(const std::string& map_kernel,
   }
   ok = finish(op, false) && ok;
   release(op);
-  if (ok) {
-    const hipMemcpyKind kind = is_device_pointer(out_result) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    ok = hipMemcpy(out_result, dev_value_, out_elem_size, kind) == hipSuccess;
-  }
+  if (ok) ok = copy_out(out_result, dev_value_, out_elem_size);
   if (!ok) report(who, last_error_text());
   return ok;
 }
@@ -628,7 +637,7 @@ size_t KernelLauncher::launch_argminmax(const std::string& kernel_name, void* da
   bool ok = resolve(data, count * elem_size, true, op);
   uint64_t idx = count;
   if (ok) ok = run_argminmax(*ctx_, d->dtype, op.dev, count, want_max, want_last, dev_word_) == PX_OK;
-  if (ok) ok = hipMemcpy(&idx, dev_word_, 8, hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok) ok = copy_out(&idx, dev_word_, 8);
   finish(op, false);
   release(op);
   return ok ? static_cast<size_t>(idx) : count;
@@ -649,7 +658,7 @@ size_t KernelLauncher::launch_find(const std::string& kernel_name, void* data, s
   bool ok = resolve(data, count * elem_size, true, op);
   uint64_t idx = count;
   if (ok) ok = run_find(*ctx_, p, op.dev, count, dev_word_) == PX_OK;
-  if (ok) ok = hipMemcpy(&idx, dev_word_, 8, hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok) ok = copy_out(&idx, dev_word_, 8);
   finish(op, false);
   release(op);
   return ok ? static_cast<size_t>(idx) : count;
@@ -663,7 +672,7 @@ size_t KernelLauncher::launch_mismatch(const std::string& kernel_name, void* a, 
   bool ok = resolve(a, count * elem_size, true, oa) && resolve(b, count * elem_size, true, ob);
   uint64_t idx = count;
   if (ok) ok = run_mismatch(*ctx_, d->dtype, oa.dev, ob.dev, count, dev_word_) == PX_OK;
-  if (ok) ok = hipMemcpy(&idx, dev_word_, 8, hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok) ok = copy_out(&idx, dev_word_, 8);
   finish(oa, false);
   release(oa);
   release(ob);

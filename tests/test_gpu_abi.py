@@ -212,3 +212,24 @@ def test_registered_descriptor_lookup(gpu_ctx):
     lib.parallax_kernel_register(b"my copy_if key:flags", w, n)
     k = lib.parallax_kernel_lookup(b"my copy_if key:flags")
     assert k
+
+
+def test_abi_results_to_device_pointers_back_to_back():
+    """Reduce results written to device memory by consecutive C-ABI calls: each result
+    word is copied out on the launcher's stream before the next call's kernel reuses the
+    launcher's device word (a copy ordered only on the null stream could read the next
+    call's value)."""
+    kf, ki = load(make_desc(C.OP_REDUCE, PX_F32)), load(make_desc(C.OP_REDUCE, PX_I64))
+    rng = np.random.default_rng(17)
+    f = rng.integers(-100, 100, 1 << 20).astype(np.float32)  # exact sums in f32
+    i = rng.integers(-(1 << 40), 1 << 40, 1 << 20).astype(np.int64)
+    df, di = DeviceBuffer.from_array(f), DeviceBuffer.from_array(i)
+    res = DeviceBuffer(16 * 32)
+    for r in range(32):
+        lib.parallax_reduce(kf, df.ptr, f.size, 4, res.addr(16 * r))
+        lib.parallax_reduce(ki, di.ptr, i.size, 8, res.addr(16 * r + 8))
+    raw = res.download(np.uint8, 16 * 32)
+    got_f = raw.reshape(32, 16)[:, :4].copy().view(np.float32).ravel()
+    got_i = raw.reshape(32, 16)[:, 8:].copy().view(np.int64).ravel()
+    assert np.all(got_f == np.float32(f.astype(np.int64).sum()))
+    assert np.all(got_i == pxo.reduce(i))
