@@ -459,3 +459,45 @@ def test_search_empty(gpu_ctx):
     assert _search(gpu_ctx, lib.pxd_find, ctypes.byref(d), p, 0) == 0
     assert _search(gpu_ctx, lib.pxd_mismatch, PX_F32, p, p, 0) == 0
     assert _search(gpu_ctx, lib.pxd_argminmax, PX_F32, p, 0, 0, 0) == 0
+
+def test_beyond_2p32_elements(gpu_ctx):
+    """n = 2^32 + 4099 int32 elements (17 GB, filled on the device): reduce, inclusive
+    scan and copy_if across the 32-bit index boundary, where compaction counts are 64-bit
+    (two status words per tile).  Closed forms: every element is 0x01010101 except 20
+    zeros straddling index 2^32."""
+    n = (1 << 32) + 4099
+    v = 0x01010101
+    src = lib.pxd_malloc(n * 4 + 64)
+    dst = lib.pxd_malloc(n * 4 + 64)
+    word = lib.pxd_malloc(64)
+    assert src and dst and word
+    try:
+        check(lib.pxd_memset(src, 1, n * 4), "memset")
+        z0 = (1 << 32) - 10
+        check(lib.pxd_memset(src + 4 * z0, 0, 20 * 4), "memset")
+
+        def at(p, i, t=np.uint32):
+            out = np.zeros(1, t)
+            check(lib.pxd_memcpy_d2h(out.ctypes.data, p + i * out.itemsize, out.itemsize), "d2h")
+            return out[0]
+
+        check(lib.pxd_reduce(gpu_ctx.h, PX_U32, src, n, word), "reduce")
+        gpu_ctx.sync()
+        assert at(word, 0) == ((n - 20) * v) % (1 << 32)
+
+        check(lib.pxd_inclusive_scan(gpu_ctx.h, PX_U32, src, dst, n), "scan")
+        gpu_ctx.sync()
+        for i in (0, 1, 12_345, z0 - 1, z0, z0 + 19, z0 + 20, (1 << 32), n - 1):
+            kept_before = i + 1 - max(0, min(i + 1, z0 + 20) - z0)  # nonzero elements in [0, i]
+            assert at(dst, i) == (kept_before * v) % (1 << 32), i
+
+        nz = make_desc(C.OP_FLAGS, PX_U32, fn=C.PRED_NONZERO)
+        check(lib.pxd_copy_if(gpu_ctx.h, ctypes.byref(nz), src, dst, n, word), "copy_if")
+        gpu_ctx.sync()
+        assert at(word, 0, np.uint64) == n - 20
+        for i in (0, z0 - 1, z0, (1 << 32) - 1, n - 21):
+            assert at(dst, i) == v, i
+    finally:
+        lib.pxd_free(src)
+        lib.pxd_free(dst)
+        lib.pxd_free(word)
