@@ -227,8 +227,11 @@ bool valid_spec(const SegSpec& sp) {
 }
 
 // Upload segment bounds + tile tables as one blob and fill a segmented geometry.
+// `extra` (optional) is appended to the same upload, its device copy returned in
+// *extra_dev (one upload per call: a second one would reuse the same device buffer).
 int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, pxk::TileCfg k, const void* in,
-             pxk::SegGeom& g, TileTables& tt) {
+             pxk::SegGeom& g, TileTables& tt, const std::vector<uint64_t>* extra = nullptr,
+             const uint64_t** extra_dev = nullptr) {
   const uint32_t nseg = sp.nseg;
   const uint32_t tile = pxk::tile_elems(dtype, k);
   const uint32_t vec = static_cast<uint32_t>(16 / pxk::dtype_size(dtype));
@@ -240,10 +243,13 @@ int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, pxk::TileCfg k, con
   const size_t b_toff = (nseg + 1) * sizeof(uint32_t);
   const size_t o_tseg = (o_toff + b_toff + 15) & ~size_t(15);
   const size_t b_tseg = static_cast<size_t>(tt.ntiles) * sizeof(uint32_t);
-  std::vector<unsigned char> blob(o_tseg + b_tseg);
-  std::memcpy(blob.data(), sp.lo, b64);
-  std::memcpy(blob.data() + o_hi, sp.hi, b64);
-  if (sp.out) std::memcpy(blob.data() + o_out, sp.out, b64);
+  const size_t o_extra = (o_tseg + b_tseg + 15) & ~size_t(15);
+  const size_t b_extra = extra ? extra->size() * sizeof(uint64_t) : 0;
+  std::vector<unsigned char> blob(o_extra + b_extra);
+  if (b64) std::memcpy(blob.data(), sp.lo, b64);
+  if (b64) std::memcpy(blob.data() + o_hi, sp.hi, b64);
+  if (sp.out && b64) std::memcpy(blob.data() + o_out, sp.out, b64);
+  if (b_extra) std::memcpy(blob.data() + o_extra, extra->data(), b_extra);
   std::memcpy(blob.data() + o_toff, tt.tile_off.data(), b_toff);
   if (b_tseg) std::memcpy(blob.data() + o_tseg, tt.tile_seg.data(), b_tseg);
   void* dev = nullptr;
@@ -257,6 +263,7 @@ int seg_geom(StreamContext& c, const SegSpec& sp, int dtype, pxk::TileCfg k, con
   g.seg_out = sp.out ? reinterpret_cast<const uint64_t*>(d + o_out) : nullptr;
   g.tile_off = reinterpret_cast<const uint32_t*>(d + o_toff);
   g.tile_seg = reinterpret_cast<const uint32_t*>(d + o_tseg);
+  if (extra_dev) *extra_dev = reinterpret_cast<const uint64_t*>(d + o_extra);
   g.n = 0;
   g.ntiles = tt.ntiles;
   g.nseg = nseg;
@@ -441,15 +448,49 @@ int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const S
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
+  // Segments of at most kShortSeg elements take one wave each (seg_scan_short); a
+  // look-back tile would spend a block's fixed work on them (B1, 16,384 f32 segments:
+  // the 9,053 of at most 1,024 elements cost 86 us of a 706 us call as tiles).
+  constexpr uint64_t kShortSeg = 1024;
+  std::vector<uint64_t> llo, lhi, lout, shrt;  // shrt: lo[], hi[], out[] of the short ones
+  std::vector<uint32_t> sidx;
+  for (uint32_t s = 0; s < sp.nseg; ++s) {
+    if (sp.hi[s] - sp.lo[s] <= kShortSeg) {
+      sidx.push_back(s);
+    } else {
+      llo.push_back(sp.lo[s]);
+      lhi.push_back(sp.hi[s]);
+      if (sp.out) lout.push_back(sp.out[s]);
+    }
+  }
+  const uint32_t nshort = static_cast<uint32_t>(sidx.size());
+  shrt.resize(static_cast<size_t>(nshort) * (sp.out ? 3 : 2));
+  for (uint32_t j = 0; j < nshort; ++j) {
+    shrt[j] = sp.lo[sidx[j]];
+    shrt[nshort + j] = sp.hi[sidx[j]];
+    if (sp.out) shrt[2 * static_cast<size_t>(nshort) + j] = sp.out[sidx[j]];
+  }
+  SegSpec lsp;
+  lsp.lo = llo.data();
+  lsp.hi = lhi.data();
+  lsp.out = sp.out ? lout.data() : nullptr;
+  lsp.nseg = static_cast<uint32_t>(llo.size());
   pxk::SegGeom g;
   TileTables tt;
-  if (tile.block == 0) tile = seg_tile_default(PX_OP_SCAN, dtype, sp);
-  int rc = seg_geom(c, sp, dtype, tile, in, g, tt);
+  if (tile.block == 0) tile = seg_tile_default(PX_OP_SCAN, dtype, lsp.nseg ? lsp : sp);
+  const uint64_t* sdev = nullptr;
+  int rc = seg_geom(c, lsp, dtype, tile, in, g, tt, &shrt, &sdev);
   if (rc) return rc;
-  if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0))
-    return PX_ERR_NOMEM;
-  if (!c.note_status_launch()) return PX_ERR_DEVICE;
-  return dev_err(pxk::scan(dtype, in, out, g, false, nullptr, c.scratch(), c.stream()), "seg_scan");
+  if (g.ntiles) {
+    if (!c.reserve(static_cast<size_t>(g.ntiles) * (pxk::carry_bytes(dtype) / 4), 0))
+      return PX_ERR_NOMEM;
+    if (!c.note_status_launch()) return PX_ERR_DEVICE;
+    rc = dev_err(pxk::scan(dtype, in, out, g, false, nullptr, c.scratch(), c.stream()), "seg_scan");
+    if (rc) return rc;
+  }
+  return dev_err(pxk::seg_scan_short(dtype, in, out, sdev, sdev + nshort, sp.out ? sdev + 2 * static_cast<size_t>(nshort) : nullptr,
+                                     nshort, c.scratch(), c.stream()),
+                 "seg_scan");
 }
 
 int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void* out,

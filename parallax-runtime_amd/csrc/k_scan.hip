@@ -283,4 +283,44 @@ hipError_t scan(int dtype, const void* in, void* out, const SegGeom& g, bool exc
   return hipErrorInvalidValue;
 }
 
+namespace {
+// Inclusive scan of short segments: one wave per segment, 64 elements per step (a wave
+// scan in the accumulator type plus the running carry), segments taken grid-stride.  A
+// look-back tile would spend a block's fixed work on a handful of elements.
+template <class T>
+__global__ __launch_bounds__(256) void k_seg_scan_short(const T* __restrict__ in, T* __restrict__ out,
+                                                        const uint64_t* lo, const uint64_t* hi,
+                                                        const uint64_t* out_lo, uint32_t nshort) {
+  using A = typename Traits<T>::A;
+  constexpr uint32_t WPB = 256 / kWave;
+  const int lane = lane_id();
+  const uint32_t nwaves = gridDim.x * WPB;
+  for (uint32_t sg = blockIdx.x * WPB + threadIdx.x / kWave; sg < nshort; sg += nwaves) {  // wave-uniform
+    const uint64_t b = lo[sg], e = hi[sg];
+    const uint64_t ob = out_lo ? out_lo[sg] : b;
+    A carry = acc_zero<A>();
+    for (uint64_t p = b; p < e; p += kWave) {
+      const uint64_t i = p + lane;
+      const A x = i < e ? to_acc(in[i]) : acc_zero<A>();
+      const A incl = acc_add(carry, wave_incl_scan(x));
+      if (i < e) out[ob + (i - b)] = from_acc<T>(incl);
+      carry = read_lane(incl, kWave - 1);
+    }
+  }
+}
+}  // namespace
+
+hipError_t seg_scan_short(int dtype, const void* in, void* out, const uint64_t* lo, const uint64_t* hi,
+                          const uint64_t* out_lo, uint32_t nshort, const Scratch& s, hipStream_t stream) {
+  if (nshort == 0) return hipSuccess;
+  const uint32_t want = (nshort + 3) / 4;  // four waves (segments) per block
+  const uint32_t grid = want < s.max_blocks ? want : s.max_blocks;
+  PXK_DISPATCH(dtype, {
+    k_seg_scan_short<T><<<grid, 256, 0, stream>>>(static_cast<const T*>(in), static_cast<T*>(out), lo, hi,
+                                                  out_lo, nshort);
+    return hipGetLastError();
+  });
+  return hipErrorInvalidValue;
+}
+
 }  // namespace pxk

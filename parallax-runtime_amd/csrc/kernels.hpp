@@ -133,6 +133,10 @@ hipError_t map_reduce(const px_desc& map, const void* in, uint64_t n, void* out_
 // ---- scan (k_scan.hip): inclusive, or exclusive with `init` (host bytes of one element)
 hipError_t scan(int dtype, const void* in, void* out, const SegGeom& g, bool exclusive,
                 const void* init, const Scratch& s, hipStream_t stream);
+// Inclusive scan of segments [lo[i], hi[i]) written from out_lo[i] (lo[i] when out_lo is
+// null), one wave per segment: for segments of at most a few hundred elements.
+hipError_t seg_scan_short(int dtype, const void* in, void* out, const uint64_t* lo, const uint64_t* hi,
+                          const uint64_t* out_lo, uint32_t nshort, const Scratch& s, hipStream_t stream);
 
 // ---- compaction (k_compact.hip)
 //   mode PX_SCATTER_COMPACT: kept elements of segment s to out + seg_out[s] (or
