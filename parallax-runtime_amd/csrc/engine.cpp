@@ -516,9 +516,9 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
-  // Segments longer than one bitonic LDS tile go to the segmented radix sort (when the
-  // radix path is enabled and every index fits 32 bits); the rest stay on the bitonic
-  // path, which sorts each in one launch.
+  // Segments longer than seg_radix_min_len (1,024 4-byte / 512 8-byte keys) go to the
+  // segmented radix sort (when the radix path is enabled and every index fits 32 bits);
+  // the rest stay on the bitonic path, which sorts each in one launch.
   const uint64_t long_min = pxk::seg_radix_min_len(dtype);
   const uint32_t rtile = pxk::seg_radix_tile(dtype);
   uint64_t span = 0;

@@ -804,8 +804,14 @@ uint64_t radix_status_words(int dtype, uint64_t n) {
 int radix_passes(int dtype) { return static_cast<int>(dtype_size(dtype)); }
 
 uint64_t seg_radix_min_len(int dtype) {
-  // one LDS tile of the bitonic path: shorter segments sort there in one launch
-  return dtype_size(dtype) == 4 ? (1ull << 14) : (1ull << 13);
+  if (tune().seg_radix_min) return tune().seg_radix_min;
+  // Segments longer than this take the segmented radix sort, even though such a segment
+  // fills only part of its tile: the bitonic tile sort of mid-size segments is slower.
+  // Measured on 4,096 segments, lengths log-uniform [16, 2^18] (us, narrow / full-range
+  // keys): u64 with the threshold at 8192 2051 / 4820, 2048 1956 / 4757, 512 1913 / 4674,
+  // 256 1913 / 4647, 64 1924 / 4707; u32 at 16384 1542 / 1958, 1024 1516 / 1920, 512
+  // 1512 / 1927.
+  return dtype_size(dtype) == 4 ? 1024 : 512;
 }
 
 uint32_t seg_radix_tile(int dtype) {

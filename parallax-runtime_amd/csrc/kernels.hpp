@@ -63,6 +63,7 @@ struct Tune {
   int radix_block = 0;        // threads per radix digit-pass block (256, 512, 1024; 0: by size)
   int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 2)
   int seg_hist_bpc = 0;       // segmented radix histogram blocks per CU (0: default, 4)
+  uint64_t seg_radix_min = 0;  // segments longer than this use the segmented radix sort (0: one bitonic LDS tile)
 };
 Tune& tune();
 // Compiled tile shapes of the look-back kernels: op PX_OP_SCAN (scans) or PX_OP_FLAGS
@@ -191,7 +192,7 @@ hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipSt
 // the second and third copies (indices up to the largest segment end; the third is
 // seg.tmp2) and seg.seg_cnt (both set by the caller inside partials,
 // seg_radix_cnt_bytes()).
-uint64_t seg_radix_min_len(int dtype);  // shorter segments stay on the bitonic path
+uint64_t seg_radix_min_len(int dtype);  // segments up to this length stay on the bitonic path
 uint32_t seg_radix_tile(int dtype);
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg);
 hipError_t seg_radix_sort(int dtype, void* data, uint64_t span, const RadixSegs& seg,
