@@ -520,7 +520,6 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
   // segmented radix sort (when the radix path is enabled and every index fits 32 bits);
   // the rest stay on the bitonic path, which sorts each in one launch.
   const uint64_t long_min = pxk::seg_radix_min_len(dtype);
-  const uint32_t rtile = pxk::seg_radix_tile(dtype);
   uint64_t span = 0;
   for (uint32_t s = 0; s < sp.nseg; ++s) span = std::max<uint64_t>(span, sp.hi[s]);
   const bool radix = pxk::tune().sort_radix_min > 0 && span < (1ull << 32) - (1ull << 20);
@@ -532,12 +531,14 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
     bhi[s] = sp.hi[s];
     if (radix && len > long_min) {
       bhi[s] = sp.lo[s];  // not on the bitonic path
-      const uint32_t ls = static_cast<uint32_t>(rlo.size());
       rlo.push_back(sp.lo[s]);
       rhi.push_back(sp.hi[s]);
-      rfirst.push_back(static_cast<uint32_t>(rtseg.size()));
-      for (uint64_t k = 0; k < (len + rtile - 1) / rtile; ++k) rtseg.push_back(ls);
     }
+  }
+  const uint32_t rtile = pxk::seg_radix_tile(dtype, rlo.data(), rhi.data(), static_cast<uint32_t>(rlo.size()));
+  for (uint32_t ls = 0; ls < rlo.size(); ++ls) {
+    rfirst.push_back(static_cast<uint32_t>(rtseg.size()));
+    for (uint64_t k = 0; k < (rhi[ls] - rlo[ls] + rtile - 1) / rtile; ++k) rtseg.push_back(ls);
   }
   pxk::SegSortPlan p;
   if (!pxk::seg_sort_plan(dtype, blo.data(), bhi.data(), sp.nseg, p)) return PX_ERR_ARG;
@@ -579,6 +580,7 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
     if (!c.note_status_launch()) return PX_ERR_DEVICE;
   const pxk::Scratch sc = c.scratch();
   pxk::RadixSegs g;
+  g.tile = rtile;
   g.ntiles = static_cast<uint32_t>(rtseg.size());
   g.nseg = nl;
   g.seg_lo = reinterpret_cast<const uint64_t*>(d + o_r);

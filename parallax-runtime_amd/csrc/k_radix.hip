@@ -238,18 +238,22 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 }
 
 // ---------------------------------------------------------------- segment geometry
-// Digit-pass block of the segmented sort: 512 threads, two blocks per CU (8K-key tiles
-// for 8-byte keys): a segment just above the radix threshold fills less of a smaller
-// tile.  Measured on 4,096 segments, lengths log-uniform [16, 2^18] (us, narrow /
-// full-range keys): u64 1024 1923 / 4673, 512 1890 / 4627, 256 2183 / 5562; u32 1024
-// 1499 / 1917, 512 1403 / 1799, 256 1589 / 2049.
-constexpr int kSegBlock = 512;
+// Digit-pass blocks of the segmented sort: 1024 threads (one per CU, 16K-key tiles) or
+// 512 (two per CU, 8K-key tiles), chosen per call by seg_radix_tile().  A segment just
+// above the radix threshold fills less of a smaller tile; a long segment pays a larger
+// tile's look-back less often.  Measured (us): 4,096 segments, lengths log-uniform
+// [16, 2^18], narrow / full-range keys: u64 1024 1923 / 4673, 512 1890 / 4627, 256
+// 2183 / 5562; u32 1024 1499 / 1917, 512 1403 / 1799, 256 1589 / 2049.  Equal segments,
+// u32 / u64: 4 x 2^22 1024 410 / 871, 512 494 / 1051; 16 x 2^20 1024 402 / 856, 512
+// 467 / 976; 64 x 2^16 1024 165 / 330, 512 234 / 483.
+constexpr int kSegBlockSmall = 512;
+constexpr int kSegBlockLarge = 1024;
 
 // Tile t of the long segments covers [tbase, tend).
 template <class T>
 __device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
                                          uint32_t& tbase, uint32_t& tend) {
-  constexpr uint32_t TILE = kSegBlock * kKptSeg<T>;
+  const uint32_t TILE = g.tile;
   seg = g.tile_seg[tile];
   const uint32_t lo = static_cast<uint32_t>(g.seg_lo[seg]);
   const uint32_t hi = static_cast<uint32_t>(g.seg_hi[seg]);
@@ -763,6 +767,9 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
   if (g.ntiles == 0) return hipSuccess;
   T* data = static_cast<T*>(data_v);
   T* tmp = static_cast<T*>(s.partials);
+  if (g.tile != static_cast<uint32_t>(kSegBlockLarge * kKptSeg<T>) &&
+      g.tile != static_cast<uint32_t>(kSegBlockSmall * kKptSeg<T>))
+    return hipErrorInvalidValue;
   if (span >= (1ull << 32) || static_cast<uint64_t>(g.ntiles) * kBins > s.status_words ||
       s.partial_bytes < 2 * span * sizeof(T) || !tmp || !g.tmp2 || !g.seg_cnt)
     return hipErrorInvalidValue;
@@ -775,9 +782,13 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
   const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, hist_per_cu * cus));
   k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, g, s.ctl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, cus * (1024 / kSegBlock)));
+  const bool large = g.tile == static_cast<uint32_t>(kSegBlockLarge * kKptSeg<T>);
+  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, large ? cus : 2 * cus));
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
-    k_radix_pass<T, kSegBlock, true><<<grid, kSegBlock, 0, stream>>>(data, tmp, span, 8 * d, g, s.status, s.ctl);
+    if (large)
+      k_radix_pass<T, kSegBlockLarge, true><<<grid, kSegBlockLarge, 0, stream>>>(data, tmp, span, 8 * d, g, s.status, s.ctl);
+    else
+      k_radix_pass<T, kSegBlockSmall, true><<<grid, kSegBlockSmall, 0, stream>>>(data, tmp, span, 8 * d, g, s.status, s.ctl);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -819,8 +830,17 @@ uint64_t seg_radix_min_len(int dtype) {
   return dtype_size(dtype) == 4 ? 1024 : 512;
 }
 
-uint32_t seg_radix_tile(int dtype) {
-  return static_cast<uint32_t>(kSegBlock * (dtype_size(dtype) == 4 ? kKptSeg<uint32_t> : kKptSeg<uint64_t>));
+uint32_t seg_radix_tile(int dtype, const uint64_t* lo, const uint64_t* hi, uint32_t n) {
+  const uint64_t kpt = dtype_size(dtype) == 4 ? kKptSeg<uint32_t> : kKptSeg<uint64_t>;
+  const uint64_t small = kSegBlockSmall * kpt, large = kSegBlockLarge * kpt;
+  if (tune().seg_radix_block) return static_cast<uint32_t>(tune().seg_radix_block * kpt);
+  uint64_t ps = 0, pl = 0;  // tile slots at each size
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t len = hi[i] - lo[i];
+    ps += (len + small - 1) / small * small;
+    pl += (len + large - 1) / large * large;
+  }
+  return static_cast<uint32_t>(pl * 20 <= ps * 21 ? large : small);
 }
 
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg) {

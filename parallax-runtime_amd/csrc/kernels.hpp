@@ -63,7 +63,8 @@ struct Tune {
   int radix_block = 0;        // threads per radix digit-pass block (256, 512, 1024; 0: by size)
   int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 2)
   int seg_hist_bpc = 0;       // segmented radix histogram blocks per CU (0: default, 4)
-  uint64_t seg_radix_min = 0;  // segments longer than this use the segmented radix sort (0: one bitonic LDS tile)
+  uint64_t seg_radix_min = 0;  // segments longer than this use the segmented radix sort (0: default, 1,024 / 512 keys)
+  int seg_radix_block = 0;     // segmented radix digit-pass block, 512 or 1024 (0: chosen per call)
 };
 Tune& tune();
 // Compiled tile shapes of the look-back kernels: op PX_OP_SCAN (scans) or PX_OP_FLAGS
@@ -172,6 +173,7 @@ hipError_t sort(int dtype, void* data, uint64_t n, hipStream_t stream, const Lau
 // Long segments of a batched sort, sorted by the radix kernels: tiles of
 // seg_radix_tile() keys that never span two segments, in segment order.
 struct RadixSegs {
+  uint32_t tile = 0;                    // keys per tile: seg_radix_tile()
   uint32_t ntiles = 0;                  // tiles of the long segments
   uint32_t nseg = 0;                    // long segments
   const uint32_t* tile_seg = nullptr;   // tile -> long-segment index
@@ -193,7 +195,10 @@ hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipSt
 // seg.tmp2) and seg.seg_cnt (both set by the caller inside partials,
 // seg_radix_cnt_bytes()).
 uint64_t seg_radix_min_len(int dtype);  // segments up to this length stay on the bitonic path
-uint32_t seg_radix_tile(int dtype);
+// Keys per tile for these long segments ([lo[i], hi[i]), i < n): 1024-thread digit-pass
+// blocks (16K keys per tile) unless their tiles hold over 5 % more slots than 512-thread
+// ones (8K), i.e. unless many segments fill a small part of a large tile.
+uint32_t seg_radix_tile(int dtype, const uint64_t* lo, const uint64_t* hi, uint32_t n);
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg);
 hipError_t seg_radix_sort(int dtype, void* data, uint64_t span, const RadixSegs& seg,
                           const Scratch& s, hipStream_t stream);

@@ -86,6 +86,7 @@ Tune load() {
     else if (k == "radix_hist_bpc" && v >= 1 && v <= 16) t.radix_hist_bpc = v;
     else if (k == "seg_hist_bpc" && v >= 1 && v <= 16) t.seg_hist_bpc = v;
     else if (k == "seg_radix_min" && v >= 1) t.seg_radix_min = static_cast<uint64_t>(v);
+    else if (k == "seg_radix_block" && (v == 512 || v == 1024)) t.seg_radix_block = v;
   }
   for (int d = 0; d < PX_NUM_DTYPES; ++d) {
     TileCfg a{scan.block ? scan.block : t.scan[d].block, scan.k ? scan.k : t.scan[d].k};

@@ -88,11 +88,16 @@ def test_seg_sort(gpu_ctx, dt):
 
 
 
-def radix_seg_input(dt, seed, kind):
+MIXED_LENS = [5, 20_000, 0, 8_192, 8_193, 16_384, 16_385, 3, 150_001, 40_000, 1, 70_000, 9_000, 33_000]
+# Long segments that fill 16K-key tiles (one ragged): 1024-thread digit-pass blocks; the
+# mixed lengths above hold 14 % more slots in 16K tiles than in 8K ones: 512-thread blocks.
+LARGE_LENS = [5, 65_536, 131_072, 16_384, 0, 49_152, 300, 32_769, 98_304]
+
+
+def radix_seg_input(dt, seed, kind, lens=MIXED_LENS):
     """Short and long segments mixed (long ones take the segmented radix path)."""
     rng = np.random.default_rng(seed)
-    lens = np.array([5, 20_000, 0, 8_192, 8_193, 16_384, 16_385, 3, 150_001, 40_000, 1, 70_000, 9_000, 33_000],
-                    dtype=np.int64)
+    lens = np.array(lens, dtype=np.int64)
     off = datagen.offsets_from_lengths(lens)
     n = int(off[-1])
     t = NP_DTYPES[dt]
@@ -116,10 +121,11 @@ def radix_seg_input(dt, seed, kind):
     return a, off
 
 
+@pytest.mark.parametrize("lens", ["mixed", "large"])
 @pytest.mark.parametrize("kind", ["full", "narrow", "byte", "equal"])
 @pytest.mark.parametrize("dt", ALL)
-def test_seg_sort_radix(gpu_ctx, dt, kind):
-    a, off = radix_seg_input(dt, 60 + dt, kind)
+def test_seg_sort_radix(gpu_ctx, dt, kind, lens):
+    a, off = radix_seg_input(dt, 60 + dt, kind, MIXED_LENS if lens == "mixed" else LARGE_LENS)
     d = DeviceBuffer.from_array(a)
     check(lib.pxb_seg_sort(gpu_ctx.h, dt, d.ptr, u64p(off), off.size - 1), "seg_sort")
     gpu_ctx.sync()
