@@ -81,8 +81,11 @@ enum px_code_kind {
 
 /* Compaction scatter modes (w[4] of a PX_OP_SCATTER descriptor). */
 enum px_scatter_mode {
-  PX_SCATTER_COMPACT = 0,   /* copy_if / remove_if / unique: kept elements only */
-  PX_SCATTER_PARTITION = 1  /* every element: kept first, the rest after num_true */
+  PX_SCATTER_COMPACT = 0,          /* copy_if / remove_if / unique: kept elements only */
+  PX_SCATTER_PARTITION = 1,        /* every element, in std::partition's order (the partition
+                                      funnel's fallback): kept first, num_true returned */
+  PX_SCATTER_STABLE_PARTITION = 2  /* every element: kept first, the rest after num_true,
+                                      both in input order (std::stable_partition) */
 };
 
 /* Predicates (w[5] of a PX_OP_FLAGS / PX_OP_FIND / count descriptor).  `c0`, `c1`

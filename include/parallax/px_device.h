@@ -100,7 +100,10 @@ int pxd_sort(pxd_ctx_t ctx, int dtype, void* d_data, size_t n);
  * holds; *d_count (device uint64) receives the kept count. */
 int pxd_copy_if(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void* d_out, size_t n,
                 uint64_t* d_count);
-/* Stable partition: kept first, the rest after; *d_num_true receives the split. */
+/* Partition in std::partition's order (libstdc++'s two-pointer exchange: the partition
+ * funnel's CPU fallback), kept first; *d_num_true receives the split.  d_in == d_out
+ * partitions in place.  The stable order is scatter mode PX_SCATTER_STABLE_PARTITION of
+ * parallax_copy_if. */
 int pxd_partition(pxd_ctx_t ctx, const px_desc* flags, const void* d_in, void* d_out, size_t n,
                   uint64_t* d_num_true);
 int pxd_transform(pxd_ctx_t ctx, const px_desc* map, const void* d_in, void* d_out, size_t n,
@@ -145,7 +148,8 @@ void parallax_transform_reduce(parallax_kernel_t map_kernel, parallax_kernel_t r
  * the fastest is captured (PARALLAX_AUTOTUNE=0 keeps the size-based default;
  * PARALLAX_TUNE_CACHE=<file> keeps choices across processes).  Creation runs the call
  * several times: d_out and d_counts are overwritten; an in-place call (d_in == d_out) is
- * timed into a scratch output, so its input is intact when the plan is returned. */
+ * timed into a scratch output and a sort warms up on a scratch copy of its keys, so the
+ * caller's data is intact when the plan is returned. */
 typedef struct pxb_plan* pxb_plan_t;
 pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flags,
                            const void* d_in, void* d_out, const uint64_t* h_offsets, size_t nseg,

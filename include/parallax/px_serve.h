@@ -49,11 +49,12 @@ typedef struct pxs_request {
 } pxs_request;
 
 typedef struct pxs_stats {
-  uint64_t submitted, completed, batches, rejected;
-  double p50_ms, p99_ms, mean_ms, max_ms;  /* arrival -> completion latency */
+  uint64_t submitted, completed, batches, rejected;  /* rejected: refused by pxs_submit */
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* arrival -> completion latency (successes) */
   double first_arrival_s, last_completion_s;
   double req_per_s;                        /* completed / (last completion - first arrival) */
   size_t pool_bytes, pool_peak_bytes;
+  uint64_t failed;                         /* submitted requests that completed with an error */
 } pxs_stats;
 
 void pxs_config_default(pxs_config* cfg);
@@ -64,17 +65,25 @@ void pxs_destroy(pxs_server_t s);
 void* pxs_alloc(pxs_server_t s, size_t bytes);
 void pxs_free(pxs_server_t s, void* d_ptr);
 
-/* Queue a request; returns its id (>= 0) or a negative PX_E* code.  The arrival time is
- * the time of this call. */
+/* Queue a request; returns its id (>= 0) or a negative PX_E* code: PX_EARG for a bad
+ * request (input outside the pool or not aligned to its element size from the pool
+ * base; bytes shared between an in-place sort and another request not yet complete),
+ * PX_ENOMEM when its output (a sort: its scratch) could never fit in the pool beside the
+ * client's inputs.  The arrival time is the time of this call. */
 int64_t pxs_submit(pxs_server_t s, const pxs_request* req);
-/* Wait for completion (timeout_ms < 0: forever).  0 = done, 1 = timed out. */
+/* Wait for completion (timeout_ms < 0: no limit).  Returns 0 when the request completed,
+ * 1 on time-out, the request's negative PX_E* code when it failed, and PX_ENOMEM when it
+ * waits for pool memory that only the client can free (nothing is running and outputs
+ * that were not released fill the pool: release some and wait again). */
 int pxs_wait(pxs_server_t s, int64_t id, double timeout_ms);
 /* Device pointer of the output (reduce: one element; scan/copy_if: n elements; sort: the
- * input), the kept count (copy_if) and the latency in ms. */
+ * input), the kept count (copy_if) and the latency in ms.  Returns the request's status:
+ * 0, or its negative PX_E* code when it failed (the output is then undefined). */
 int pxs_result(pxs_server_t s, int64_t id, void** d_out, uint64_t* count, double* latency_ms);
 /* Frees the request's output (and its bookkeeping). */
 int pxs_release(pxs_server_t s, int64_t id);
-/* Blocks until every submitted request has completed. */
+/* Blocks until every submitted request has completed (0), or returns PX_ENOMEM when the
+ * remaining ones wait for pool memory that only the client can free. */
 int pxs_drain(pxs_server_t s);
 int pxs_get_stats(pxs_server_t s, pxs_stats* out);
 /* Clears the latency record (keeps buffers and requests). */

@@ -188,18 +188,16 @@ __attribute__((noinline)) std::size_t device_copy_if(const T* in, T* out, std::s
   return m;
 }
 
+// remove_if / partition / unique compact in place: the runtime stages host memory itself
+// and its compaction kernels are safe with input == output (every tile writes only below
+// its own end, after every earlier tile has read its input), so no temporary is needed
+// and no size limit applies.  partition comes back in std::partition's order, so the
+// device and the host loop give the same bytes.
 template <class T, class Pred>
 __attribute__((noinline)) std::size_t device_remove_if(T* data, std::size_t n, Pred pred) {
   static CompactKernels k = compact_kernels(__PRETTY_FUNCTION__);
-  if (k.ok() && std::is_empty_v<Pred>) {
-    T* tmp = static_cast<T*>(parallax_arena_alloc(n * sizeof(T), alignof(T)));
-    if (tmp) {
-      const std::size_t m = parallax_copy_if(k.f, k.s, k.a, k.x, data, tmp, n, sizeof(T), is_fp<T>);
-      std::memcpy(data, tmp, m * sizeof(T));
-      parallax_arena_free(tmp);
-      return m;
-    }
-  }
+  if (k.ok() && std::is_empty_v<Pred>)
+    return parallax_copy_if(k.f, k.s, k.a, k.x, data, data, n, sizeof(T), is_fp<T>);
   std::size_t m = 0;
   for (std::size_t i = 0; i < n; ++i)
     if (!pred(data[i])) data[m++] = data[i];
@@ -209,30 +207,15 @@ __attribute__((noinline)) std::size_t device_remove_if(T* data, std::size_t n, P
 template <class T, class Pred>
 __attribute__((noinline)) std::size_t device_partition(T* data, std::size_t n, Pred pred) {
   static CompactKernels k = compact_kernels(__PRETTY_FUNCTION__);
-  if (k.ok() && std::is_empty_v<Pred>) {
-    T* tmp = static_cast<T*>(parallax_arena_alloc(n * sizeof(T), alignof(T)));
-    if (tmp) {
-      const std::size_t m = parallax_copy_if(k.f, k.s, k.a, k.x, data, tmp, n, sizeof(T), is_fp<T>);
-      std::memcpy(data, tmp, n * sizeof(T));  // kept first, then the rest (both stable)
-      parallax_arena_free(tmp);
-      return m;
-    }
-  }
+  if (k.ok() && std::is_empty_v<Pred>)
+    return parallax_copy_if(k.f, k.s, k.a, k.x, data, data, n, sizeof(T), is_fp<T>);
   return static_cast<std::size_t>(std::partition(data, data + n, pred) - data);
 }
 
 template <class T>
 __attribute__((noinline)) std::size_t device_unique(T* data, std::size_t n) {
   static CompactKernels k = compact_kernels(__PRETTY_FUNCTION__);
-  if (k.ok()) {
-    T* tmp = static_cast<T*>(parallax_arena_alloc(n * sizeof(T), alignof(T)));
-    if (tmp) {
-      const std::size_t m = parallax_copy_if(k.f, k.s, k.a, k.x, data, tmp, n, sizeof(T), is_fp<T>);
-      std::memcpy(data, tmp, m * sizeof(T));
-      parallax_arena_free(tmp);
-      return m;
-    }
-  }
+  if (k.ok()) return parallax_copy_if(k.f, k.s, k.a, k.x, data, data, n, sizeof(T), is_fp<T>);
   return static_cast<std::size_t>(std::unique(data, data + n) - data);
 }
 

@@ -460,6 +460,15 @@ void pxo_seg_inclusive_scan(int dtype, const void* in, void* out, const uint64_t
   })
 }
 
+void pxo_seg_scan_exact(int dtype, const void* in, const uint64_t* offsets, size_t nseg,
+                        double* out_val, double* out_abs) {
+  const size_t es = (dtype == PX_I32 || dtype == PX_U32 || dtype == PX_F32) ? 4 : 8;
+  const char* p = static_cast<const char*>(in);
+  for (size_t s = 0; s < nseg; ++s)
+    pxo_scan_exact(dtype, p + offsets[s] * es, offsets[s + 1] - offsets[s], 0, nullptr,
+                   out_val + offsets[s], out_abs ? out_abs + offsets[s] : nullptr);
+}
+
 void pxo_seg_sort(int dtype, void* data, const uint64_t* offsets, size_t nseg) {
   PXO_DISPATCH(dtype, {
     T* p = static_cast<T*>(data);

@@ -9,11 +9,14 @@
  * the oracle can be driven with exactly the descriptor the GPU path receives.
  *
  * Pinning (what each function is checked against besides retyped formulas):
- *   reduce / scans / sort / copy_if / unique / partition:
- *       the reference's known-answer tests (tests/unit/test_*.cpp), the reference
- *       fallback itself compiled from include/parallax/stdpar.hpp into oracle/_ref
- *       (tests/test_oracle_vs_ref.py), and closed forms (arithmetic series, sorted
- *       permutations, modular sums).
+ *   reduce / scans / sort / copy_if / unique:
+ *       the reference's known-answer tests (tests/unit/test_*.cpp) and closed forms
+ *       (arithmetic series, sorted permutations, modular sums, numpy equivalents).
+ *   partition: libstdc++'s two-pointer exchange restated as a loop and in closed form
+ *       (tests/test_oracle.py), both against this function's std::partition.
+ *   The reference's fallback cannot be built on its own here: its funnel header needs
+ *   the reference runtime library (Vulkan), so none of this is compared with a build
+ *   of the reference.
  *   sum_exact / scan_exact: double-double sums checked against Python's math.fsum
  *       (exactly rounded) and integer-valued closed forms.
  *   predicates / maps: brute-force tables in tests/test_oracle.py against Python
@@ -76,6 +79,9 @@ void pxo_seg_sum_exact(int dtype, const void* in, const uint64_t* offsets, size_
                        double* hi, double* abs_sum);
 void pxo_seg_inclusive_scan(int dtype, const void* in, void* out, const uint64_t* offsets,
                             size_t nseg);
+/* per-segment pxo_scan_exact (inclusive) into out_val / out_abs at the segment's offsets */
+void pxo_seg_scan_exact(int dtype, const void* in, const uint64_t* offsets, size_t nseg,
+                        double* out_val, double* out_abs);
 void pxo_seg_sort(int dtype, void* data, const uint64_t* offsets, size_t nseg);
 /* per-segment std::unique of in -> out packed segment after segment; out_offsets[nseg+1] */
 size_t pxo_seg_unique(int dtype, const void* in, void* out, const uint64_t* offsets,

@@ -1,9 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/libpx_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg as the checker.  The product library never uses it.
-Also binds the reference's own CPU fallback (oracle/_ref/libref_fallback.so, built from
-include/parallax/stdpar.hpp of the reference by oracle/build_ref.sh) when present.
+cpu_baseline leg as the checker (and the timed CPU baseline).  The product library never
+uses it.
 """
 import ctypes
 import pathlib
@@ -12,7 +11,6 @@ import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
 ORACLE_PATH = HERE / "libpx_oracle.so"
-REF_PATH = HERE / "_ref" / "libref_fallback.so"
 
 _NP = {0: np.int32, 1: np.uint32, 2: np.int64, 3: np.uint64, 4: np.float32, 5: np.float64}
 _CODE = {np.dtype(v): k for k, v in _NP.items()}
@@ -59,6 +57,7 @@ def _load():
         ("pxo_seg_reduce", None, [i, vp, u64p, sz, vp]),
         ("pxo_seg_sum_exact", None, [i, vp, u64p, sz, vp, vp]),
         ("pxo_seg_inclusive_scan", None, [i, vp, vp, u64p, sz]),
+        ("pxo_seg_scan_exact", None, [i, vp, u64p, sz, vp, vp]),
         ("pxo_seg_sort", None, [i, vp, u64p, sz]),
         ("pxo_seg_unique", sz, [i, vp, vp, u64p, sz, u64p]),
     ]:
@@ -214,6 +213,15 @@ def seg_inclusive_scan(a, off):
     return out
 
 
+def seg_scan_exact(a, off):
+    """Per-segment inclusive-scan reference values and prefix sums of |x| (float64)."""
+    off, op = _off(off)
+    val = np.empty(a.size, dtype=np.float64)
+    ab = np.empty(a.size, dtype=np.float64)
+    lib.pxo_seg_scan_exact(code(a), _p(a), op, off.size - 1, _p(val), _p(ab))
+    return val, ab
+
+
 def seg_sort(a, off):
     off, op = _off(off)
     out = a.copy()
@@ -229,11 +237,3 @@ def seg_unique(a, off):
     k = lib.pxo_seg_unique(code(a), _p(a), _p(out), op, off.size - 1,
                            oo.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     return out[:k], oo
-
-
-# ---- the reference's own fallback (built from its stdpar.hpp), when present
-def load_ref():
-    if not REF_PATH.exists():
-        return None
-    R = ctypes.CDLL(str(REF_PATH))
-    return R

@@ -331,20 +331,30 @@ pxb_plan_t pxb_plan_create(pxd_ctx_t ctx, int op, int dtype, const px_desc* flag
   const parallax::SegSpec sp = contiguous(h_offsets, nseg);
   int rc = PX_EOK;
   // An in-place scan / compaction runs its tuning and warm-up calls into a scratch
-  // output, so the caller's input is intact when the plan is returned.
+  // output, and a sort warms up on a scratch copy of its keys (at the same offset within
+  // a 16-byte line), so the caller's data is intact when the plan is returned.
   const int dt = (op == PX_OP_FLAGS && flags) ? static_cast<int>(flags->dtype) : dtype;
   void* warm_out = d_out;
   void* tmp = nullptr;
-  if ((op == PX_OP_SCAN || op == PX_OP_FLAGS) && d_in == d_out && nseg > 0 && dt >= 0 &&
-      dt < PX_NUM_DTYPES) {
-    const size_t bytes = static_cast<size_t>(h_offsets[nseg]) * pxk::dtype_size(dt) + 16;
-    if (hipMalloc(&tmp, bytes) != hipSuccess) {
+  const bool warm_copy = op == PX_OP_SORT || ((op == PX_OP_SCAN || op == PX_OP_FLAGS) && d_in == d_out);
+  if (warm_copy && nseg > 0 && dt >= 0 && dt < PX_NUM_DTYPES && d_out) {
+    const size_t data_bytes = static_cast<size_t>(h_offsets[nseg]) * pxk::dtype_size(dt);
+    const size_t mis = reinterpret_cast<uintptr_t>(d_out) & 15u;
+    if (hipMalloc(&tmp, data_bytes + 16) != hipSuccess) {
       (void)hipGetLastError();
       parallax::set_last_error(PX_ENOMEM, "plan scratch output");
       pxb_plan_destroy(p);
       return nullptr;
     }
-    warm_out = tmp;
+    warm_out = op == PX_OP_SORT ? static_cast<char*>(tmp) + mis : tmp;
+    if (op == PX_OP_SORT && data_bytes &&
+        hipMemcpyAsync(warm_out, d_out, data_bytes, hipMemcpyDeviceToDevice, p->c->stream()) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(tmp);
+      parallax::set_last_error(PX_EDEVICE, "plan warm-up copy");
+      pxb_plan_destroy(p);
+      return nullptr;
+    }
   }
   // 0) tile shape: timed on this batch (scan / compaction / reduce)
   if (op == PX_OP_REDUCE || op == PX_OP_SCAN || op == PX_OP_FLAGS) {

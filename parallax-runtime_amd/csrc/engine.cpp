@@ -91,9 +91,16 @@ bool StreamContext::grow(void** p, size_t* cap, size_t need, bool zero) {
   *p = nullptr;
   *cap = 0;
   if (hipMalloc(p, ncap) != hipSuccess) {
+    // the doubled size did not fit: try the size asked for
     (void)hipGetLastError();
     *p = nullptr;
-    return false;
+    ncap = (need + 4095) & ~size_t(4095);
+    if (hipMalloc(p, ncap) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      set_last_error(PX_ERR_NOMEM, "scratch allocation of " + std::to_string(ncap) + " bytes failed");
+      return false;
+    }
   }
   if (zero && hipMemset(*p, 0, ncap) != hipSuccess) return false;
   *cap = ncap;
@@ -334,27 +341,61 @@ int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* ou
   const size_t words = n >= (1ull << 32) ? 2 : 1;
   if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
-  return dev_err(pxk::compact(flags, PX_SCATTER_COMPACT, in, out, g, count_dev, nullptr,
+  return dev_err(pxk::compact(flags, pxk::kModeKept, in, out, g, count_dev, nullptr,
                               c.scratch(), c.stream()),
                  "compact");
 }
 
+// Partition in std::partition's order (stable == false) or the stable order.
+//   std order: count -> num_true; (out != in: copy in -> out); compaction of the indices
+//   of the misplaced elements into the partials; swap pass on out.  Any in / out overlap
+//   is fine only for in == out.
+//   stable order: count, then one compaction pass writing every element; in place it
+//   compacts into the partials and copies back (rejected elements land ahead of unread
+//   input otherwise).
 int run_partition(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
-                  uint64_t* num_true_dev) {
+                  uint64_t* num_true_dev, bool stable) {
   if (!valid_dtype(flags.dtype) || !num_true_dev || (n && (!in || !out))) return PX_ERR_ARG;
+  if (flags.fn == PX_PRED_UNIQUE) {
+    set_last_error(PX_ERR_ARG, "partition by adjacent-unique flags is undefined");
+    return PX_ERR_ARG;
+  }
   ++g_stats.compact;
   ++g_stats.launches;
   if (n == 0) return zero_dev(c, num_true_dev, sizeof(uint64_t));
-  if (!c.reserve(0, static_cast<size_t>(c.cu_count()) * 8 * 16)) return PX_ERR_NOMEM;
+  const size_t es = pxk::dtype_size(flags.dtype);
+  const bool wide = n >= (1ull << 32);
+  const size_t idx_bytes = wide ? 8 : 4;
+  // partials: [0, 256) count word of the index pass, then the index list (or the stable
+  // pass's in-place staging copy); the counting pass uses the front before either
+  const size_t body = stable ? ((in == out) ? n * es : 0) : n * idx_bytes;
+  const size_t need = std::max<size_t>(static_cast<size_t>(c.cu_count()) * 8 * 16, 256 + body);
+  const pxk::TileCfg tile = stable ? plain_tile(PX_OP_FLAGS, flags.dtype) : pxk::misplaced_tile(flags.dtype, n);
+  const pxk::SegGeom g = plain_geom(n, flags.dtype, tile, in);
+  const size_t words = wide ? 2 : 1;
+  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, need)) return PX_ERR_NOMEM;
   int rc = dev_err(pxk::count_if(flags, in, n, num_true_dev, c.scratch(), c.stream()), "count");
   if (rc) return rc;
-  const pxk::SegGeom g = plain_geom(n, flags.dtype, plain_tile(PX_OP_FLAGS, flags.dtype), in);
-  const size_t words = n >= (1ull << 32) ? 2 : 1;
-  if (!c.reserve(static_cast<size_t>(g.ntiles) * words, 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
-  return dev_err(pxk::compact(flags, PX_SCATTER_PARTITION, in, out, g, nullptr, num_true_dev,
-                              c.scratch(), c.stream()),
+  const pxk::Scratch sc = c.scratch();
+  unsigned char* part = static_cast<unsigned char*>(sc.partials);
+  if (stable) {
+    void* dst = (in == out) ? static_cast<void*>(part + 256) : out;
+    rc = dev_err(pxk::compact(flags, pxk::kModeStable, in, dst, g, nullptr, num_true_dev, sc, c.stream()),
                  "partition");
+    if (rc || in != out) return rc;
+    return dev_err(hipMemcpyAsync(out, part + 256, n * es, hipMemcpyDeviceToDevice, c.stream()), "partition copy");
+  }
+  if (in != out) {
+    rc = dev_err(hipMemcpyAsync(out, in, n * es, hipMemcpyDeviceToDevice, c.stream()), "partition copy");
+    if (rc) return rc;
+  }
+  uint64_t* count2 = reinterpret_cast<uint64_t*>(part);
+  rc = dev_err(pxk::compact(flags, pxk::kModeMisplaced, in, part + 256, g, count2, num_true_dev, sc,
+                            c.stream()),
+               "partition");
+  if (rc) return rc;
+  return dev_err(pxk::swap_pairs(flags.dtype, out, part + 256, wide, count2, n, sc, c.stream()), "partition swap");
 }
 
 int run_transform(StreamContext& c, const px_desc& d, const void* in, void* out, uint64_t n,
@@ -506,49 +547,80 @@ int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void
   if (rc) return rc;
   if (!c.reserve(static_cast<size_t>(g.ntiles), 0)) return PX_ERR_NOMEM;
   if (!c.note_status_launch()) return PX_ERR_DEVICE;
-  return dev_err(pxk::compact(flags, PX_SCATTER_COMPACT, in, out, g, counts_dev, nullptr,
+  return dev_err(pxk::compact(flags, pxk::kModeKept, in, out, g, counts_dev, nullptr,
                               c.scratch(), c.stream()),
                  "seg_compact");
 }
 
-int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
+namespace {
+// Which segments of a batched sort take the segmented radix sort, and where each sits in
+// the packed scratch buffers (segments longer than seg_radix_min_len -- 1,024 4-byte /
+// 512 8-byte keys -- and shorter than 2^31 keys, when the radix path is enabled).
+struct LongSegs {
+  std::vector<uint32_t> idx;  // segment index in the batch
+  std::vector<uint64_t> lo, hi, pack;
+  uint64_t packed = 0;  // elements of one scratch buffer
+};
+LongSegs long_segments(int dtype, const SegSpec& sp) {
+  LongSegs L;
+  if (pxk::tune().sort_radix_min == 0) return L;
+  const uint64_t long_min = pxk::seg_radix_min_len(dtype);
+  const uint64_t vec = 16 / pxk::dtype_size(dtype);
+  for (uint32_t s = 0; s < sp.nseg; ++s) {
+    const uint64_t len = sp.hi[s] - sp.lo[s];
+    if (len <= long_min || len >= (1ull << 31)) continue;
+    // same position within a 16-byte line as in the array
+    const uint64_t at = (L.packed + vec - 1) / vec * vec + sp.lo[s] % vec;
+    L.idx.push_back(s);
+    L.lo.push_back(sp.lo[s]);
+    L.hi.push_back(sp.hi[s]);
+    L.pack.push_back(at);
+    L.packed = at + len;
+  }
+  return L;
+}
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+size_t seg_sort_scratch_bytes(int dtype, const SegSpec& sp) {
+  if (!valid_dtype(dtype) || !valid_spec(sp)) return 0;
+  const LongSegs L = long_segments(dtype, sp);
+  if (L.lo.empty()) return 0;
+  return 2 * round256(L.packed * pxk::dtype_size(dtype)) +
+         pxk::seg_radix_cnt_bytes(dtype, static_cast<uint32_t>(L.lo.size()));
+}
+
+int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp, void* ext_scratch,
+                 size_t ext_bytes) {
   if (!valid_dtype(dtype) || !data || !valid_spec(sp)) return PX_ERR_ARG;
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
   ++g_stats.launches;
-  // Segments longer than seg_radix_min_len (1,024 4-byte / 512 8-byte keys) go to the
-  // segmented radix sort (when the radix path is enabled and every index fits 32 bits);
-  // the rest stay on the bitonic path, which sorts each in one launch.
-  const uint64_t long_min = pxk::seg_radix_min_len(dtype);
-  uint64_t span = 0;
-  for (uint32_t s = 0; s < sp.nseg; ++s) span = std::max<uint64_t>(span, sp.hi[s]);
-  const bool radix = pxk::tune().sort_radix_min > 0 && span < (1ull << 32) - (1ull << 20);
-  std::vector<uint64_t> blo(sp.nseg), bhi(sp.nseg), rlo, rhi;
+  // Long segments go to the segmented radix sort (long_segments); the rest stay on the
+  // bitonic path, which sorts each in one launch.
+  const LongSegs L = long_segments(dtype, sp);
+  std::vector<uint64_t> blo(sp.nseg), bhi(sp.nseg);
   std::vector<uint32_t> rfirst, rtseg;
   for (uint32_t s = 0; s < sp.nseg; ++s) {
-    const uint64_t len = sp.hi[s] - sp.lo[s];
     blo[s] = sp.lo[s];
     bhi[s] = sp.hi[s];
-    if (radix && len > long_min) {
-      bhi[s] = sp.lo[s];  // not on the bitonic path
-      rlo.push_back(sp.lo[s]);
-      rhi.push_back(sp.hi[s]);
-    }
   }
-  const uint32_t rtile = pxk::seg_radix_tile(dtype, rlo.data(), rhi.data(), static_cast<uint32_t>(rlo.size()));
-  for (uint32_t ls = 0; ls < rlo.size(); ++ls) {
+  for (uint32_t s : L.idx) bhi[s] = blo[s];  // not on the bitonic path
+  const uint32_t nl = static_cast<uint32_t>(L.lo.size());
+  const uint32_t rtile = pxk::seg_radix_tile(dtype, L.lo.data(), L.hi.data(), nl);
+  for (uint32_t ls = 0; ls < nl; ++ls) {
     rfirst.push_back(static_cast<uint32_t>(rtseg.size()));
-    for (uint64_t k = 0; k < (rhi[ls] - rlo[ls] + rtile - 1) / rtile; ++k) rtseg.push_back(ls);
+    for (uint64_t k = 0; k < (L.hi[ls] - L.lo[ls] + rtile - 1) / rtile; ++k) rtseg.push_back(ls);
   }
   pxk::SegSortPlan p;
   if (!pxk::seg_sort_plan(dtype, blo.data(), bhi.data(), sp.nseg, p)) return PX_ERR_ARG;
   // blob: interleaved bitonic bounds (lo, hi) as u64, the bitonic block tables (u32),
-  // then the radix tables: seg_lo, seg_hi (u64), seg_first, tile_seg (u32)
-  const uint32_t nl = static_cast<uint32_t>(rlo.size());
+  // then the radix tables: seg_lo, seg_hi, seg_pack (u64), seg_first, tile_seg (u32)
   const size_t b_b = static_cast<size_t>(sp.nseg) * 2 * sizeof(uint64_t);
   const size_t o_r = (b_b + p.tables.size() * sizeof(uint32_t) + 15) & ~size_t(15);
   const size_t o_rhi = o_r + nl * sizeof(uint64_t);
-  const size_t o_rfirst = o_rhi + nl * sizeof(uint64_t);
+  const size_t o_rpack = o_rhi + nl * sizeof(uint64_t);
+  const size_t o_rfirst = o_rpack + nl * sizeof(uint64_t);
   const size_t o_rtseg = o_rfirst + nl * sizeof(uint32_t);
   std::vector<unsigned char> blob(o_rtseg + rtseg.size() * sizeof(uint32_t));
   uint64_t* bounds = reinterpret_cast<uint64_t*>(blob.data());
@@ -559,16 +631,22 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
   if (!p.tables.empty())
     std::memcpy(blob.data() + b_b, p.tables.data(), p.tables.size() * sizeof(uint32_t));
   if (nl) {
-    std::memcpy(blob.data() + o_r, rlo.data(), nl * sizeof(uint64_t));
-    std::memcpy(blob.data() + o_rhi, rhi.data(), nl * sizeof(uint64_t));
+    std::memcpy(blob.data() + o_r, L.lo.data(), nl * sizeof(uint64_t));
+    std::memcpy(blob.data() + o_rhi, L.hi.data(), nl * sizeof(uint64_t));
+    std::memcpy(blob.data() + o_rpack, L.pack.data(), nl * sizeof(uint64_t));
     std::memcpy(blob.data() + o_rfirst, rfirst.data(), nl * sizeof(uint32_t));
     std::memcpy(blob.data() + o_rtseg, rtseg.data(), rtseg.size() * sizeof(uint32_t));
   }
-  const size_t buf_bytes = (span * pxk::dtype_size(dtype) + 255) & ~size_t(255);
-  const size_t cnt_off = 2 * buf_bytes;  // scratch copy, third buffer, then the counts
-  if (nl && !c.reserve(static_cast<size_t>(rtseg.size()) * 256,
-                       cnt_off + pxk::seg_radix_cnt_bytes(dtype, nl)))
+  // scratch: second buffer, third buffer (the long segments packed), then the counts;
+  // from the caller (the server's pool, counted against its budget) or the context
+  const size_t buf_bytes = round256(L.packed * pxk::dtype_size(dtype));
+  const size_t scratch_bytes = nl ? 2 * buf_bytes + pxk::seg_radix_cnt_bytes(dtype, nl) : 0;
+  if (nl && !c.reserve(static_cast<size_t>(rtseg.size()) * 256, ext_scratch ? 0 : scratch_bytes))
     return PX_ERR_NOMEM;
+  if (nl && ext_scratch && ext_bytes < scratch_bytes) {
+    set_last_error(PX_ERR_ARG, "seg_sort: external scratch too small");
+    return PX_ERR_ARG;
+  }
   void* dev = nullptr;
   if (!c.upload(blob.data(), blob.size(), &dev)) return PX_ERR_NOMEM;
   unsigned char* d = static_cast<unsigned char*>(dev);
@@ -579,17 +657,20 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp) {
   for (size_t i = 0; i < pxk::dtype_size(dtype); ++i)
     if (!c.note_status_launch()) return PX_ERR_DEVICE;
   const pxk::Scratch sc = c.scratch();
+  unsigned char* base = static_cast<unsigned char*>(ext_scratch ? ext_scratch : sc.partials);
   pxk::RadixSegs g;
   g.tile = rtile;
   g.ntiles = static_cast<uint32_t>(rtseg.size());
   g.nseg = nl;
   g.seg_lo = reinterpret_cast<const uint64_t*>(d + o_r);
   g.seg_hi = reinterpret_cast<const uint64_t*>(d + o_rhi);
+  g.seg_pack = reinterpret_cast<const uint64_t*>(d + o_rpack);
   g.seg_first = reinterpret_cast<const uint32_t*>(d + o_rfirst);
   g.tile_seg = reinterpret_cast<const uint32_t*>(d + o_rtseg);
-  g.seg_cnt = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(sc.partials) + cnt_off);
-  g.tmp2 = static_cast<unsigned char*>(sc.partials) + buf_bytes;
-  return dev_err(pxk::seg_radix_sort(dtype, data, span, g, sc, c.stream()), "seg_radix_sort");
+  g.tmp_b = base;
+  g.tmp2 = base + buf_bytes;
+  g.seg_cnt = reinterpret_cast<uint32_t*>(base + 2 * buf_bytes);
+  return dev_err(pxk::seg_radix_sort(dtype, data, g, sc, c.stream()), "seg_radix_sort");
 }
 
 }  // namespace parallax

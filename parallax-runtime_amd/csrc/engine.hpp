@@ -117,8 +117,10 @@ int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::Lau
 // copy_if / remove_if / unique: count to *count_dev (device u64)
 int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
                 uint64_t* count_dev);
+// Partition of in into out (in == out: in place): std::partition's order, or with
+// stable = true the stable order (kept first, the rest after, both in input order).
 int run_partition(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,
-                  uint64_t* num_true_dev);
+                  uint64_t* num_true_dev, bool stable = false);
 int run_transform(StreamContext& c, const px_desc& d, const void* in, void* out, uint64_t n,
                   const void* captures, size_t capture_size, uint64_t host_base,
                   uint64_t dev_base, uint64_t arena_bytes);
@@ -134,7 +136,12 @@ int run_seg_reduce(StreamContext& c, int dtype, const void* in, const SegSpec& s
                    pxk::TileCfg tile = {0, 0});
 int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const SegSpec& sp,
                  pxk::TileCfg tile = {0, 0});
-int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp);
+// Batched sort.  The segmented radix sort of the long segments needs
+// seg_sort_scratch_bytes() of scratch: from ext_scratch (device memory of the caller,
+// e.g. the server's pool) when given, else from the context.
+size_t seg_sort_scratch_bytes(int dtype, const SegSpec& sp);
+int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp,
+                 void* ext_scratch = nullptr, size_t ext_bytes = 0);
 int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void* out,
                     const SegSpec& sp, uint64_t* counts_dev, pxk::TileCfg tile = {0, 0});
 // Default tile of a batched call: the largest candidate no longer than the mean

@@ -10,9 +10,16 @@
 // LDS in order, looks back for the number kept before the tile, and writes them as one
 // contiguous, coalesced run.  Bytes moved: n*s read + kept*s written.
 //
-// Partition needs the total kept count before any rejected element can be placed
-// (kept first, the rest after num_true, both stable): a counting pass writes num_true
-// to device memory and the compaction pass reads it there (no host round trip).
+// Partitions need the total kept count first: a counting pass writes num_true to device
+// memory and the next pass reads it there (no host round trip).
+//  * stable partition (kept first, the rest after num_true, both in input order): the
+//    compaction pass places rejected elements too;
+//  * std::partition's order (the fallback of the partition funnel, reference
+//    include/parallax/stdpar.hpp:369, libstdc++'s two-pointer form: the i-th predicate-
+//    false element of [0, num_true) swaps with the i-th predicate-true element of
+//    [num_true, n) counted from the end): the compaction pass writes the INDICES of the
+//    misplaced elements (false below num_true, true at or above it) in order, 2m of them,
+//    and a swap pass exchanges index i with index 2m-1-i for i < m.  In place.
 #include "dev_common.hpp"
 #include "dev_pred.hpp"
 #include "dev_tiles.hpp"
@@ -44,20 +51,26 @@ __device__ __forceinline__ void write_run(T* dst, const T* src, C count) {
   if (threadIdx.x < count - done) dst[done + threadIdx.x] = src[done + threadIdx.x];
 }
 
+// MODE kModeKept: kept elements; kModeStable: kept first, then the rest after
+// *num_true_dev; kModeMisplaced: the indices (type C) of the elements on the wrong side of
+// *num_true_dev (predicate false below it, true at or above it), in order.  The output
+// and staging element type S is T, or C for kModeMisplaced.
 template <class T, int BLOCK, int K, class C, int MODE, bool UNIQUE>
-__global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_shift, T* out,
-                                                 SegGeom g, px_desc pd, uint64_t* counts,
-                                                 const uint64_t* num_true_dev, uint64_t* status,
-                                                 Ctl* ctl) {
+__global__ __launch_bounds__(BLOCK) void k_compact(
+    const T* in_al, int64_t in_shift, std::conditional_t<MODE == kModeMisplaced, C, T>* out,
+    SegGeom g, px_desc pd, uint64_t* counts, const uint64_t* num_true_dev, uint64_t* status,
+    Ctl* ctl) {
+  using S = std::conditional_t<MODE == kModeMisplaced, C, T>;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int TILE = BLOCK * K * VEC;
   constexpr int NW = BLOCK / kWave;
   constexpr int E = K * VEC;  // elements per thread (<= 32: one flag bit each)
   static_assert(E <= 32, "flag mask holds 32 elements");
+  static_assert(TILE * sizeof(S) <= 128 * 1024, "staging fits the LDS");
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
 
-  __shared__ T s_stage[TILE];
+  __shared__ S s_stage[TILE];
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
   __shared__ C s_w[NW];
@@ -79,12 +92,15 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     C tex[K];
     C wave_off, kept;
     int lo_r, hi_r, vstart_r;
-    int64_t w0;
+    int64_t w0, lo;
   };
+  // the split of a partition (read once: the counting pass ran before on this stream)
+  const uint64_t split = (MODE == kModeMisplaced) ? *num_true_dev : 0;
 
   // Loads, flags and ranks the tile (ends with the block's counts in s_w read out).
   auto load_rank = [&](Tile& u, const TileInfo& ti) {
     u.w0 = ti.w0;
+    u.lo = ti.lo;
     const int64_t dl = ti.lo - u.w0, dh = ti.hi - u.w0;
     u.lo_r = dl < 0 ? -1 : (dl > TILE ? TILE : static_cast<int>(dl));
     u.hi_r = dh > TILE ? TILE : (dh < 0 ? 0 : static_cast<int>(dh));
@@ -139,6 +155,21 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
     } else {
       u.flags = pred.template mask<E>(u.x) & u.valid;
     }
+    if constexpr (MODE == kModeMisplaced) {
+      // position pos = w0 + rel - lo is misplaced when pred(x) == (pos >= split):
+      // flip the predicate bits of the positions below the split
+      const int64_t pr64 = static_cast<int64_t>(split) + u.lo - u.w0;
+      const int pr = pr64 < 0 ? 0 : (pr64 > TILE ? TILE : static_cast<int>(pr64));
+      uint32_t left = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int r0 = base_rel(k);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          if (r0 + j < pr) left |= 1u << (k * VEC + j);
+      }
+      u.flags = (u.flags ^ left) & u.valid;
+    }
 
     // ---- rank kept elements: per-chunk wave scans of counts
     C run = 0;
@@ -168,9 +199,12 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
       for (int j = 0; j < VEC; ++j) {
         const int e = k * VEC + j;
         if ((u.flags >> e) & 1u) {
-          s_stage[r] = u.x[e];
+          if constexpr (MODE == kModeMisplaced)
+            s_stage[r] = static_cast<S>(u.w0 + r0 + j - u.lo);
+          else
+            s_stage[r] = u.x[e];
           ++r;
-        } else if (MODE == PX_SCATTER_PARTITION) {
+        } else if (MODE == kModeStable) {
           if ((u.valid >> e) & 1u) {
             const C vi = static_cast<C>(r0 + j - u.vstart_r);  // valid index inside the tile
             s_stage[u.kept + (vi - r)] = u.x[e];
@@ -182,13 +216,13 @@ __global__ __launch_bounds__(BLOCK) void k_compact(const T* in_al, int64_t in_sh
 
   // Writes the staged run(s); kept_before = kept in the segment before the tile.
   auto write = [&](const Tile& u, const TileInfo& ti, uint64_t seg_out, uint64_t kept_before) {
-    write_run<T, BLOCK>(out + seg_out + kept_before, s_stage, u.kept);
-    if (MODE == PX_SCATTER_PARTITION) {
+    write_run<S, BLOCK>(out + seg_out + kept_before, s_stage, u.kept);
+    if constexpr (MODE == kModeStable) {
       const uint64_t num_true = *num_true_dev;
       const C sub_valid = static_cast<C>(u.hi_r > u.vstart_r ? u.hi_r - u.vstart_r : 0);
       if (sub_valid > 0) {
         const uint64_t rej_before = static_cast<uint64_t>(u.w0 + u.vstart_r - ti.lo) - kept_before;
-        write_run<T, BLOCK>(out + seg_out + num_true + rej_before, s_stage + u.kept, sub_valid - u.kept);
+        write_run<S, BLOCK>(out + seg_out + num_true + rej_before, s_stage + u.kept, sub_valid - u.kept);
       }
     }
   };
@@ -279,17 +313,21 @@ __global__ __launch_bounds__(256) void k_count(const T* in, uint64_t n, px_desc 
 }
 
 template <class T, class C, int MODE, bool UNIQUE>
-hipError_t launch_compact_t(hipStream_t stream, const T* in_al, int64_t in_shift, T* out,
+hipError_t launch_compact_t(hipStream_t stream, const T* in_al, int64_t in_shift, void* out,
                             const SegGeom& g, const px_desc& pd, uint64_t* counts,
                             const uint64_t* num_true_dev, const Scratch& s) {
+  using S = std::conditional_t<MODE == kModeMisplaced, C, T>;
+  S* o = static_cast<S*>(out);
   const TileCfg c{g.tile_block, g.tile_k};
   const uint32_t grid = lookback_grid(g.ntiles, s.max_blocks, c.block);
   if (grid == 0) return hipSuccess;
 #define PXK_CMP(B, KK)                                                                        \
-  if (c.block == B && c.k == KK) {                                                            \
-    k_compact<T, B, KK, C, MODE, UNIQUE><<<grid, B, 0, stream>>>(                             \
-        in_al, in_shift, out, g, pd, counts, num_true_dev, s.status, s.ctl);                  \
-    return hipGetLastError();                                                                 \
+  if constexpr (B * KK * 16 / sizeof(T) * sizeof(S) <= 128 * 1024) {                          \
+    if (c.block == B && c.k == KK) {                                                          \
+      k_compact<T, B, KK, C, MODE, UNIQUE><<<grid, B, 0, stream>>>(                           \
+          in_al, in_shift, o, g, pd, counts, num_true_dev, s.status, s.ctl);                  \
+      return hipGetLastError();                                                               \
+    }                                                                                         \
   }
   PXK_CMP(256, 4)
   PXK_CMP(256, 8)
@@ -312,21 +350,60 @@ hipError_t compact_t(const px_desc& pd, int mode, const void* in, void* out, con
   const size_t words = wide ? 2 : 1;
   if (static_cast<size_t>(g.ntiles) * words > s.status_words) return hipErrorInvalidValue;
   const bool uniq = (pd.fn == PX_PRED_UNIQUE);
-  T* o = reinterpret_cast<T*>(out);
-  if (mode == PX_SCATTER_PARTITION) {
-    if (g.seg_lo != nullptr || num_true_dev == nullptr) return hipErrorInvalidValue;
+  if (mode == kModeStable || mode == kModeMisplaced) {
+    if (g.seg_lo != nullptr || num_true_dev == nullptr || uniq) return hipErrorInvalidValue;
+    if (mode == kModeMisplaced) {
+      const TileCfg want = misplaced_tile(dtype_of<T>(), g.n);
+      if (g.tile_block != want.block || g.tile_k != want.k) return hipErrorInvalidValue;
+      if (wide)
+        return launch_compact_t<T, uint64_t, kModeMisplaced, false>(stream, in_al, in_shift, out, g, pd, counts, num_true_dev, s);
+      return launch_compact_t<T, uint32_t, kModeMisplaced, false>(stream, in_al, in_shift, out, g, pd, counts, num_true_dev, s);
+    }
     if (wide)
-      return launch_compact_t<T, uint64_t, PX_SCATTER_PARTITION, false>(stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
-    return launch_compact_t<T, uint32_t, PX_SCATTER_PARTITION, false>(stream, in_al, in_shift, o, g, pd, counts, num_true_dev, s);
+      return launch_compact_t<T, uint64_t, kModeStable, false>(stream, in_al, in_shift, out, g, pd, counts, num_true_dev, s);
+    return launch_compact_t<T, uint32_t, kModeStable, false>(stream, in_al, in_shift, out, g, pd, counts, num_true_dev, s);
   }
+  if (mode != kModeKept) return hipErrorInvalidValue;
   if (uniq) {
     if (wide)
-      return launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, true>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
-    return launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, true>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+      return launch_compact_t<T, uint64_t, kModeKept, true>(stream, in_al, in_shift, out, g, pd, counts, nullptr, s);
+    return launch_compact_t<T, uint32_t, kModeKept, true>(stream, in_al, in_shift, out, g, pd, counts, nullptr, s);
   }
   if (wide)
-    return launch_compact_t<T, uint64_t, PX_SCATTER_COMPACT, false>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
-  return launch_compact_t<T, uint32_t, PX_SCATTER_COMPACT, false>(stream, in_al, in_shift, o, g, pd, counts, nullptr, s);
+    return launch_compact_t<T, uint64_t, kModeKept, false>(stream, in_al, in_shift, out, g, pd, counts, nullptr, s);
+  return launch_compact_t<T, uint32_t, kModeKept, false>(stream, in_al, in_shift, out, g, pd, counts, nullptr, s);
+}
+
+// Swap pass of std::partition's order: idx[0, 2m) holds the misplaced positions in
+// increasing order (m below the split, m at or above it); position idx[i] exchanges with
+// idx[2m-1-i] for i < m.  The pairs are disjoint, so the swaps need no ordering.
+template <class T, class C>
+__global__ __launch_bounds__(256) void k_swap_pairs(T* data, const C* idx, const uint64_t* count2) {
+  const uint64_t two_m = *count2;
+  const uint64_t m = two_m / 2;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < m;
+       i += static_cast<uint64_t>(gridDim.x) * 256) {
+    const uint64_t a = idx[i];
+    const uint64_t b = idx[two_m - 1 - i];
+    const T va = data[a];
+    const T vb = data[b];
+    data[a] = vb;
+    data[b] = va;
+  }
+}
+
+template <class T>
+hipError_t swap_pairs_t(void* data, const void* idx, bool wide, const uint64_t* count2_dev,
+                        uint64_t n, const Scratch& s, hipStream_t stream) {
+  uint64_t grid = (n / 2 + 255) / 256;
+  if (grid < 1) grid = 1;
+  if (grid > s.max_blocks) grid = s.max_blocks;
+  T* d = static_cast<T*>(data);
+  if (wide)
+    k_swap_pairs<T, uint64_t><<<static_cast<uint32_t>(grid), 256, 0, stream>>>(d, static_cast<const uint64_t*>(idx), count2_dev);
+  else
+    k_swap_pairs<T, uint32_t><<<static_cast<uint32_t>(grid), 256, 0, stream>>>(d, static_cast<const uint32_t*>(idx), count2_dev);
+  return hipGetLastError();
 }
 
 template <class T>
@@ -354,6 +431,19 @@ hipError_t compact(const px_desc& pred, int mode, const void* in, void* out, con
 hipError_t count_if(const px_desc& pred, const void* in, uint64_t n, uint64_t* count_dev,
                     const Scratch& s, hipStream_t stream) {
   PXK_DISPATCH(pred.dtype, return count_t<T>(pred, in, n, count_dev, s, stream));
+  return hipErrorInvalidValue;
+}
+
+TileCfg misplaced_tile(int dtype, uint64_t n) {
+  // 1024 x 8 vectors, or 1024 x 4 when 64-bit indices of 4-byte elements would not fit
+  // the 128 KiB staging area
+  return (dtype_size(dtype) == 4 && n >= (1ull << 32)) ? TileCfg{1024, 4} : TileCfg{1024, 8};
+}
+
+hipError_t swap_pairs(int dtype, void* data, const void* idx, bool wide_idx,
+                      const uint64_t* count2_dev, uint64_t n, const Scratch& s,
+                      hipStream_t stream) {
+  PXK_DISPATCH(dtype, return swap_pairs_t<T>(data, idx, wide_idx, count2_dev, n, s, stream));
   return hipErrorInvalidValue;
 }
 

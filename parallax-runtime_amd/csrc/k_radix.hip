@@ -249,16 +249,16 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 constexpr int kSegBlockSmall = 512;
 constexpr int kSegBlockLarge = 1024;
 
-// Tile t of the long segments covers [tbase, tend).
+// Tile t of the long segments covers [tbase, tend) of its segment (positions relative
+// to the segment's first key: a segment holds fewer than 2^32 keys, the batch may not).
 template <class T>
 __device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
                                          uint32_t& tbase, uint32_t& tend) {
   const uint32_t TILE = g.tile;
   seg = g.tile_seg[tile];
-  const uint32_t lo = static_cast<uint32_t>(g.seg_lo[seg]);
-  const uint32_t hi = static_cast<uint32_t>(g.seg_hi[seg]);
-  tbase = lo + (tile - g.seg_first[seg]) * TILE;
-  tend = hi - tbase < TILE ? hi : tbase + TILE;
+  const uint32_t len = static_cast<uint32_t>(g.seg_hi[seg] - g.seg_lo[seg]);
+  tbase = (tile - g.seg_first[seg]) * TILE;
+  tend = len - tbase < TILE ? len : tbase + TILE;
 }
 
 // ---------------------------------------------------------------- one digit pass
@@ -275,7 +275,9 @@ __device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint
 // (the array), `b` and g.tmp2 so that the last one writes the array and none reads and
 // writes the same buffer: k even a -> b -> a ..; k odd (>= 3) a -> b -> tmp2 -> b ..
 // -> a; k = 1: that digit's pass and the first skipped digit's launch, which copies the
-// long segments' keys, go a -> b -> a in launch order.
+// long segments' keys, go a -> b -> a in launch order.  A segment's keys sit at
+// a + seg_lo[s] in the array and at seg_pack[s] in `b` and tmp2 (long segments packed
+// one after the other), positions inside a segment are 32-bit.
 template <class T, int BLOCK, bool SEG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_radix_pass(
     T* __restrict__ a, T* __restrict__ b, uint64_t n, int shift, RadixSegs g,
@@ -292,6 +294,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   const uint32_t ntiles = SEG ? g.ntiles : static_cast<uint32_t>((n + TILE - 1) / TILE);
   const T* __restrict__ in = a;
   T* __restrict__ out = b;
+  bool in_a = true, out_a = false;  // SEG: which side is the array (segment bases differ)
   if constexpr (SEG) {
     const uint32_t skip = ld_agent32(&ctl->radix_skip);
     const int dig = shift / 8;
@@ -304,11 +307,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       const bool first = dig == (r < s0 ? r : s0);
       in = first ? a : b;
       out = first ? b : a;
+      in_a = first;
+      out_a = !first;
       if (dig == s0) {
         for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
           uint32_t sg, tb, te;
           seg_tile<T>(g, t, sg, tb, te);
-          for (uint32_t p = tb + threadIdx.x; p < te; p += BLOCK) out[p] = in[p];
+          const T* ti = in + (in_a ? g.seg_lo[sg] : g.seg_pack[sg]);
+          T* to = out + (out_a ? g.seg_lo[sg] : g.seg_pack[sg]);
+          for (uint32_t p = tb + threadIdx.x; p < te; p += BLOCK) to[p] = ti[p];
         }
         return;
       }
@@ -317,8 +324,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       const int i = __builtin_popcount(runs & ((1u << dig) - 1u));  // this pass's run index
       T* const buf[3] = {a, b, static_cast<T*>(g.tmp2)};
       auto dst = [&](int j) { return j == k - 1 ? 0 : ((k & 1) ? 1 + (j & 1) : 1 - (j & 1)); };
-      in = buf[i == 0 ? 0 : dst(i - 1)];
-      out = buf[dst(i)];
+      const int bi = i == 0 ? 0 : dst(i - 1), bo = dst(i);
+      in = buf[bi];
+      out = buf[bo];
+      in_a = bi == 0;
+      out_a = bo == 0;
     }
   } else {
     const uint32_t skip = ld_agent32(&ctl->radix_skip);
@@ -381,14 +391,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // order, so they take the tile's last local indices [tn, TILE), which are never
     // written out (only the last tile has them, and nothing reads its counts).
     // The tile covers positions [tbase, tend) of the array.
-    uint32_t tbase, tend, first_tile = 0, seg = 0, seg_lo = 0;
+    // SEG: positions relative to the segment, through the segment's base pointers
+    uint32_t tbase, tend, first_tile = 0, seg = 0;
+    const T* __restrict__ tin = in;
+    T* __restrict__ tout = out;
     if constexpr (SEG) {
       seg = g.tile_seg[tile];
       first_tile = g.seg_first[seg];
-      seg_lo = static_cast<uint32_t>(g.seg_lo[seg]);
-      const uint32_t seg_hi = static_cast<uint32_t>(g.seg_hi[seg]);
-      tbase = seg_lo + (tile - first_tile) * static_cast<uint32_t>(TILE);
-      tend = seg_hi - tbase < static_cast<uint32_t>(TILE) ? seg_hi : tbase + static_cast<uint32_t>(TILE);
+      const uint64_t lo64 = g.seg_lo[seg], pk = g.seg_pack[seg];
+      const uint32_t len = static_cast<uint32_t>(g.seg_hi[seg] - lo64);
+      tbase = (tile - first_tile) * static_cast<uint32_t>(TILE);
+      tend = len - tbase < static_cast<uint32_t>(TILE) ? len : tbase + static_cast<uint32_t>(TILE);
+      tin = in + (in_a ? lo64 : pk);
+      tout = out + (out_a ? lo64 : pk);
     } else {
       const uint32_t n32 = static_cast<uint32_t>(n);
       tbase = tile * static_cast<uint32_t>(TILE);
@@ -401,7 +416,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 
     // ---- loads (a wave's key j of all lanes is 64 consecutive keys)
     T key[KPT];
-    const T* lp = in + first;
+    const T* lp = tin + first;
 #pragma unroll
     for (int j = 0; j < KPT; ++j) key[j] = j < nv ? __builtin_nontemporal_load(lp + j * kWave) : radix_pad<T>();
 
@@ -516,7 +531,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       }
       uint32_t bin;
       if constexpr (SEG) {
-        bin = seg_lo + sincl - scount;
+        bin = sincl - scount;
         for (int w = 0; w < wid; ++w) bin += s_bsum[w];
       } else {
         bin = bins[b];
@@ -532,12 +547,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       for (int i0 = 0; i0 < KPT; ++i0) {
         const uint32_t i = static_cast<uint32_t>(i0) * BLOCK + threadIdx.x;
         const T k = s_keys[i];
-        out[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
+        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
       }
     } else {
       for (uint32_t i = threadIdx.x; i < tn; i += BLOCK) {
         const T k = s_keys[i];
-        out[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
+        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
       }
     }
     __syncthreads();
@@ -633,7 +648,7 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n) {
 // complements; the last block marks the digits every key shares (their passes are
 // skipped) and clears the ORs for the next call.
 template <class T>
-__global__ __launch_bounds__(256) void k_segradix_hist(const T* data, RadixSegs g, Ctl* ctl) {
+__global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixSegs g, Ctl* ctl) {
   constexpr int D = sizeof(T);
   constexpr int HB = 256;
   constexpr int NWH = HB / kWave;
@@ -693,6 +708,7 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data, RadixSegs 
       if (cur != 0xffffffffu) flush(cur);
       cur = seg;
     }
+    const T* data = data_all + g.seg_lo[seg];  // positions are relative to the segment
     const uint32_t mis = static_cast<uint32_t>(((reinterpret_cast<uintptr_t>(data + tbase)) & 15u) / sizeof(T));
     uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
     if (head > tend - tbase) head = tend - tbase;
@@ -762,16 +778,16 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data, RadixSegs 
 }
 
 template <class T>
-hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, const Scratch& s,
+hipError_t seg_radix_sort_t(void* data_v, const RadixSegs& g, const Scratch& s,
                             hipStream_t stream) {
   if (g.ntiles == 0) return hipSuccess;
   T* data = static_cast<T*>(data_v);
-  T* tmp = static_cast<T*>(s.partials);
+  T* tmp = static_cast<T*>(g.tmp_b);
   if (g.tile != static_cast<uint32_t>(kSegBlockLarge * kKptSeg<T>) &&
       g.tile != static_cast<uint32_t>(kSegBlockSmall * kKptSeg<T>))
     return hipErrorInvalidValue;
-  if (span >= (1ull << 32) || static_cast<uint64_t>(g.ntiles) * kBins > s.status_words ||
-      s.partial_bytes < 2 * span * sizeof(T) || !tmp || !g.tmp2 || !g.seg_cnt)
+  if (static_cast<uint64_t>(g.ntiles) * kBins > s.status_words || !tmp || !g.tmp2 || !g.seg_cnt ||
+      !g.seg_pack)
     return hipErrorInvalidValue;
   const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
   const uint64_t ncnt = static_cast<uint64_t>(g.nseg) * sizeof(T) * kBins;
@@ -786,9 +802,9 @@ hipError_t seg_radix_sort_t(void* data_v, uint64_t span, const RadixSegs& g, con
   const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, large ? cus : 2 * cus));
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
     if (large)
-      k_radix_pass<T, kSegBlockLarge, true><<<grid, kSegBlockLarge, 0, stream>>>(data, tmp, span, 8 * d, g, s.status, s.ctl);
+      k_radix_pass<T, kSegBlockLarge, true><<<grid, kSegBlockLarge, 0, stream>>>(data, tmp, 0, 8 * d, g, s.status, s.ctl);
     else
-      k_radix_pass<T, kSegBlockSmall, true><<<grid, kSegBlockSmall, 0, stream>>>(data, tmp, span, 8 * d, g, s.status, s.ctl);
+      k_radix_pass<T, kSegBlockSmall, true><<<grid, kSegBlockSmall, 0, stream>>>(data, tmp, 0, 8 * d, g, s.status, s.ctl);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -847,9 +863,9 @@ uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg) {
   return static_cast<uint64_t>(nseg) * dtype_size(dtype) * kBins * sizeof(uint32_t);
 }
 
-hipError_t seg_radix_sort(int dtype, void* data, uint64_t span, const RadixSegs& seg,
-                          const Scratch& s, hipStream_t stream) {
-  PXK_DISPATCH(dtype, return seg_radix_sort_t<T>(data, span, seg, s, stream));
+hipError_t seg_radix_sort(int dtype, void* data, const RadixSegs& seg, const Scratch& s,
+                          hipStream_t stream) {
+  PXK_DISPATCH(dtype, return seg_radix_sort_t<T>(data, seg, s, stream));
   return hipErrorInvalidValue;
 }
 

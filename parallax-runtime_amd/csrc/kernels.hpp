@@ -8,6 +8,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "parallax/descriptor.h"
@@ -99,6 +100,16 @@ struct Scratch {
     default: break;                                              \
   }
 
+template <class T>
+constexpr int dtype_of() {
+  if constexpr (std::is_same_v<T, int32_t>) return PX_I32;
+  else if constexpr (std::is_same_v<T, uint32_t>) return PX_U32;
+  else if constexpr (std::is_same_v<T, int64_t>) return PX_I64;
+  else if constexpr (std::is_same_v<T, uint64_t>) return PX_U64;
+  else if constexpr (std::is_same_v<T, float>) return PX_F32;
+  else return PX_F64;
+}
+
 inline size_t dtype_size(int dtype) {
   switch (dtype) {
     case PX_I32: case PX_U32: case PX_F32: return 4;
@@ -141,13 +152,22 @@ hipError_t seg_scan_short(int dtype, const void* in, void* out, const uint64_t* 
                           const uint64_t* out_lo, uint32_t nshort, const Scratch& s, hipStream_t stream);
 
 // ---- compaction (k_compact.hip)
-//   mode PX_SCATTER_COMPACT: kept elements of segment s to out + seg_out[s] (or
-//     seg_lo[s]); kept count to counts[s] (counts may be null for one segment: then
-//     Ctl::result).
-//   mode PX_SCATTER_PARTITION (one segment): kept first, the rest after *num_true_dev.
+//   kModeKept: kept elements of segment s to out + seg_out[s] (or seg_lo[s]); kept count
+//     to counts[s] (counts may be null for one segment: then Ctl::result).
+//   kModeStable (one segment): kept first, the rest after *num_true_dev, both stable.
+//   kModeMisplaced (one segment, tile misplaced_tile()): the indices (uint32, or uint64
+//     from 2^32 elements) of the elements on the wrong side of *num_true_dev --
+//     predicate false below it, true at or above it -- in increasing order to out, their
+//     count to counts[0]; with swap_pairs() this gives std::partition's order in place.
+enum CompactMode { kModeKept = 0, kModeStable = 1, kModeMisplaced = 2 };
 hipError_t compact(const px_desc& pred, int mode, const void* in, void* out, const SegGeom& g,
                    uint64_t* counts, const uint64_t* num_true_dev, const Scratch& s,
                    hipStream_t stream);
+TileCfg misplaced_tile(int dtype, uint64_t n);
+// Exchanges data[idx[i]] and data[idx[2m-1-i]] for i < m, 2m = *count2_dev.
+hipError_t swap_pairs(int dtype, void* data, const void* idx, bool wide_idx,
+                      const uint64_t* count2_dev, uint64_t n, const Scratch& s,
+                      hipStream_t stream);
 // Count of predicate-true elements of one segment into *count_dev (partition's first pass).
 hipError_t count_if(const px_desc& pred, const void* in, uint64_t n, uint64_t* count_dev,
                     const Scratch& s, hipStream_t stream);
@@ -178,10 +198,12 @@ struct RadixSegs {
   uint32_t nseg = 0;                    // long segments
   const uint32_t* tile_seg = nullptr;   // tile -> long-segment index
   const uint32_t* seg_first = nullptr;  // long segment -> its first tile
-  const uint64_t* seg_lo = nullptr;     // long segment -> first element
-  const uint64_t* seg_hi = nullptr;     // long segment -> end element
+  const uint64_t* seg_lo = nullptr;     // long segment -> first element (in the array)
+  const uint64_t* seg_hi = nullptr;     // long segment -> end element (hi - lo < 2^32)
+  const uint64_t* seg_pack = nullptr;   // long segment -> first element in tmp_b / tmp2
   uint32_t* seg_cnt = nullptr;          // long segment x digit position x 256 counts
-  void* tmp2 = nullptr;                 // third buffer (same indexing as the array)
+  void* tmp_b = nullptr;                // second buffer (segments packed, seg_pack)
+  void* tmp2 = nullptr;                 // third buffer (same packing)
 };
 bool radix_sort_ok(int dtype, uint64_t n);
 uint64_t radix_status_words(int dtype, uint64_t n);
@@ -190,18 +212,17 @@ hipError_t radix_sort(int dtype, void* data, uint64_t n, const Scratch& s, hipSt
                       const LaunchHook& hook = {});
 // Segmented radix sort of the long segments: one counting pass (per-segment digit counts,
 // and the digits every key shares, whose passes are skipped on the device), then one pass
-// per digit that runs, the last writing the array.  Scratch: 256 status words per tile; partials hold
-// the second and third copies (indices up to the largest segment end; the third is
-// seg.tmp2) and seg.seg_cnt (both set by the caller inside partials,
-// seg_radix_cnt_bytes()).
+// per digit that runs, the last writing the array.  Scratch: 256 status words per tile;
+// the caller provides seg.tmp_b and seg.tmp2 (each holding the long segments packed at
+// seg.seg_pack: Σ lengths plus alignment) and seg.seg_cnt (seg_radix_cnt_bytes()).
 uint64_t seg_radix_min_len(int dtype);  // segments up to this length stay on the bitonic path
 // Keys per tile for these long segments ([lo[i], hi[i]), i < n): 1024-thread digit-pass
 // blocks (16K keys per tile) unless their tiles hold over 5 % more slots than 512-thread
 // ones (8K), i.e. unless many segments fill a small part of a large tile.
 uint32_t seg_radix_tile(int dtype, const uint64_t* lo, const uint64_t* hi, uint32_t n);
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg);
-hipError_t seg_radix_sort(int dtype, void* data, uint64_t span, const RadixSegs& seg,
-                          const Scratch& s, hipStream_t stream);
+hipError_t seg_radix_sort(int dtype, void* data, const RadixSegs& seg, const Scratch& s,
+                          hipStream_t stream);
 // Segmented sort of [seg_off[s], seg_off[s+1]) for every segment: a host plan (block
 // tables for each launch kind) uploaded by the caller, then a fixed launch sequence.
 struct SegSortPlan {

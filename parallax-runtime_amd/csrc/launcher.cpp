@@ -93,7 +93,7 @@ bool KernelLauncher::load_kernel(const std::string& name, const uint32_t* words,
   if (d.op == PX_OP_TRANSFORM && d.aux >= PX_NUM_DTYPES) return false;
   if ((d.op == PX_OP_MAP || d.op == PX_OP_TRANSFORM) && d.fn >= PX_NUM_MAPOPS) return false;
   if ((d.op == PX_OP_FLAGS || d.op == PX_OP_FIND) && d.fn >= PX_NUM_PREDS) return false;
-  if (d.op == PX_OP_SCATTER && d.aux > PX_SCATTER_PARTITION) return false;
+  if (d.op == PX_OP_SCATTER && d.aux > PX_SCATTER_STABLE_PARTITION) return false;
   kernels_[name] = d;
   if (debug_on()) std::fprintf(stderr, "[parallax] load %s op=%u dtype=%u\n", name.c_str(), d.op, d.dtype);
   return true;
@@ -597,16 +597,20 @@ bool KernelLauncher::launch_compact(const std::string& flags_kernel, const std::
   if (!size_ok(f->dtype, elem_size))
     return report("launch_compact", "elem_size does not match the descriptor"), false;
   if (count == 0) return true;
-  const bool partition = x->aux == PX_SCATTER_PARTITION;
+  const bool partition = x->aux == PX_SCATTER_PARTITION || x->aux == PX_SCATTER_STABLE_PARTITION;
   if (partition && f->fn == PX_PRED_UNIQUE)
     return report("launch_compact", "partition by adjacent-unique flags is undefined"), false;
+  // input == output compacts (or partitions) in place: one operand, staged once
+  const bool in_place = input == output;
   Operand in, out;
   bool ok = resolve(input, count * elem_size, true, in) &&
-            resolve(output, count * elem_size, false, out);
+            (in_place || resolve(output, count * elem_size, false, out));
+  Operand& dst = in_place ? in : out;
   if (ok) {
     prof_begin();
-    ok = (partition ? run_partition(*ctx_, *f, in.dev, out.dev, count, dev_word_)
-                    : run_compact(*ctx_, *f, in.dev, out.dev, count, dev_word_)) == PX_OK;
+    ok = (partition ? run_partition(*ctx_, *f, in.dev, dst.dev, count, dev_word_,
+                                    x->aux == PX_SCATTER_STABLE_PARTITION)
+                    : run_compact(*ctx_, *f, in.dev, dst.dev, count, dev_word_)) == PX_OK;
     prof_end(partition ? "partition" : "copy_if", f->dtype);
   }
   uint64_t kept = 0;
@@ -614,11 +618,11 @@ bool KernelLauncher::launch_compact(const std::string& flags_kernel, const std::
   if (ok) ok = hipStreamSynchronize(ctx_->stream()) == hipSuccess;
   if (ok) {
     // only the written part of the output migrates back (the rest keeps host contents)
-    out.bytes = (partition ? count : kept) * elem_size;
-    ok = finish(out, true);
+    dst.bytes = (partition ? count : kept) * elem_size;
+    ok = finish(dst, true);
   }
   release(in);
-  release(out);
+  if (!in_place) release(out);
   if (!ok) {
     report("launch_compact", last_error_text());
     return false;

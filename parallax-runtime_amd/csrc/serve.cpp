@@ -3,18 +3,27 @@
 //
 // One worker thread owns the schedule:
 //   1. completions: a slot whose event has fired completes every request of its batch
-//      (latency = completion seen - arrival);
+//      with the batch's status (latency = completion seen - arrival);
 //   2. admission, in arrival order: a request is admitted once its output fits in the
 //      pool (the budget); the head of the queue blocks the ones behind it, so nothing
-//      starves;
+//      starves.  A request whose output could never fit beside the client's resident
+//      inputs fails with PX_ENOMEM instead of blocking the queue;
 //   3. dispatch: every free slot takes the oldest admitted request and all admitted
 //      requests of the same (op, dtype, predicate) -- one segmented launch over
-//      segments that stay where they are in the pool (no gather copies).
+//      segments that stay where they are in the pool (no gather copies).  Scratch a batch
+//      needs (copy_if counts, the segmented radix sort's buffers) comes from the same
+//      pool, so it counts against the budget too.
 // Slots are streams with their own scratch, so batches of different groups overlap on
 // the device while new requests keep arriving.
+//
+// When nothing runs and the queue head waits for memory only the client can free
+// (outputs it has not released), the server is stalled: pxs_wait on a request that is
+// not running and pxs_drain return PX_ENOMEM instead of blocking forever; a release or
+// free clears the state and the worker tries again.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -40,11 +49,17 @@ double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+size_t round_pool(size_t b) {
+  const size_t a = parallax::UnifiedArena::kDefaultAlignment;
+  return (b + a - 1) / a * a;
+}
+
 enum class St { PENDING, READY, RUNNING, DONE };
 
 struct Req {
   pxs_request r;
   St st = St::PENDING;
+  int status = PX_EOK;  // of a DONE request: PX_EOK or the failure's PX_E* code
   double t_arr = 0, t_done = 0;
   void* d_out = nullptr;  // server-allocated output (null for sort)
   uint64_t count = 0;
@@ -55,8 +70,10 @@ struct Slot {
   std::unique_ptr<StreamContext> ctx;
   hipEvent_t ev = nullptr;
   bool busy = false;
+  int rc = PX_EOK;               // status the batch completes with
   std::vector<int64_t> ids;
-  void* counts = nullptr;       // device counts of a copy_if batch (pool)
+  void* counts = nullptr;        // device counts of a copy_if batch (pool)
+  void* scratch = nullptr;       // segmented radix buffers of a sort batch (pool)
   uint64_t* pin_counts = nullptr;
   size_t pin_cap = 0;
 };
@@ -68,6 +85,24 @@ bool same_group(const Req& a, const Req& b) {
          a.r.flags.c0 == b.r.flags.c0 && a.r.flags.c1 == b.r.flags.c1;
 }
 
+size_t out_bytes(const pxs_request& r, size_t es) {
+  if (r.op == PX_OP_REDUCE) return es;
+  if (r.op == PX_OP_SCAN || r.op == PX_OP_FLAGS) return r.n * es;
+  return 0;
+}
+
+// Scratch bytes a sort of [lo, lo + n) takes from the pool (segmented radix buffers).
+size_t sort_scratch(int dtype, uint64_t lo, uint64_t n) {
+  const uint64_t hi = lo + n;
+  SegSpec sp;
+  sp.lo = &lo;
+  sp.hi = &hi;
+  sp.nseg = 1;
+  return parallax::seg_sort_scratch_bytes(dtype, sp);
+}
+
+enum LaunchResult { kLaunched, kNoMemory, kFailed };
+
 }  // namespace
 
 struct pxs_server {
@@ -78,20 +113,70 @@ struct pxs_server {
   std::condition_variable cv_work, cv_done;
   std::deque<int64_t> pending, ready;
   std::unordered_map<int64_t, Req> reqs;
+  std::unordered_map<void*, size_t> client_allocs;  // pxs_alloc'd inputs
+  size_t client_bytes = 0;
   std::vector<Slot> slots;
   std::thread worker;
   bool stop = false;
+  bool stalled = false;
+  std::atomic<bool> wake{false};  // new work or freed memory: re-evaluate the schedule
   int64_t next_id = 0;
-  uint64_t submitted = 0, completed = 0, batches = 0, rejected = 0;
+  uint64_t submitted = 0, completed = 0, failed = 0, batches = 0, rejected = 0;
   std::vector<double> lat_ms;
   double first_arr = 0, last_done = 0;
   size_t peak = 0;
 
   void loop();
-  bool launch(Slot& s, const std::vector<int64_t>& ids);
+  int launch(Slot& s, const std::vector<int64_t>& ids);
+  void finish_slot(Slot& s, double t);
+  void fail(int64_t id, int code, double t);
+  // Bytes the pool could give once every server-held and releasable block is free.
+  size_t attainable() const {
+    return pool.capacity() > client_bytes ? pool.capacity() - client_bytes : 0;
+  }
+  bool any_busy() const {
+    for (const auto& s : slots)
+      if (s.busy) return true;
+    return false;
+  }
 };
 
-bool pxs_server::launch(Slot& s, const std::vector<int64_t>& ids) {
+void pxs_server::fail(int64_t id, int code, double t) {
+  Req& q = reqs[id];
+  q.st = St::DONE;
+  q.status = code;
+  q.t_done = t;
+  ++failed;
+}
+
+void pxs_server::finish_slot(Slot& s, double t) {
+  for (size_t i = 0; i < s.ids.size(); ++i) {
+    Req& q = reqs[s.ids[i]];
+    q.st = St::DONE;
+    q.status = s.rc;
+    q.t_done = t;
+    if (s.rc == PX_EOK) {
+      if (s.counts) q.count = s.pin_counts[i];
+      lat_ms.push_back((t - q.t_arr) * 1e3);
+      ++completed;
+    } else {
+      ++failed;
+    }
+  }
+  if (s.rc == PX_EOK) last_done = t;
+  if (s.counts) pool.deallocate(s.counts);
+  if (s.scratch) pool.deallocate(s.scratch);
+  s.counts = s.scratch = nullptr;
+  s.ids.clear();
+  s.busy = false;
+  s.rc = PX_EOK;
+}
+
+// Launches one batch on a free slot.  kNoMemory: the pool has no room for the batch's
+// scratch now (nothing was queued).  kFailed: the batch failed and its requests are
+// complete (failed) already, or complete with the failure once work queued before the
+// failure has drained (the slot is then busy with rc set).
+int pxs_server::launch(Slot& s, const std::vector<int64_t>& ids) {
   const Req& r0 = reqs[ids.front()];
   const size_t es = r0.es;
   const char* base = static_cast<const char*>(pool.device_base());
@@ -109,40 +194,62 @@ bool pxs_server::launch(Slot& s, const std::vector<int64_t>& ids) {
   sp.out = out.data();
   sp.nseg = nseg;
   void* b = pool.device_base();
+  s.counts = s.scratch = nullptr;
+  size_t scratch_bytes = 0;
+  if (r0.r.op == PX_OP_SORT) {
+    sp.out = nullptr;
+    scratch_bytes = parallax::seg_sort_scratch_bytes(r0.r.dtype, sp);
+    if (scratch_bytes && !(s.scratch = pool.allocate(scratch_bytes))) return kNoMemory;
+  } else if (r0.r.op == PX_OP_FLAGS) {
+    if (!(s.counts = pool.allocate(nseg * sizeof(uint64_t)))) return kNoMemory;
+  }
+  const double t = now_s();
+  auto fail_now = [&](int code) {
+    if (s.counts) pool.deallocate(s.counts);
+    if (s.scratch) pool.deallocate(s.scratch);
+    s.counts = s.scratch = nullptr;
+    for (int64_t id : ids) fail(id, code, t);
+    return kFailed;
+  };
+  if (r0.r.op == PX_OP_FLAGS && s.pin_cap < nseg) {
+    if (s.pin_counts) (void)hipHostFree(s.pin_counts);
+    s.pin_counts = nullptr;
+    s.pin_cap = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&s.pin_counts), nseg * sizeof(uint64_t),
+                      hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail_now(PX_ENOMEM);
+    }
+    s.pin_cap = nseg;
+  }
   int rc = PX_EOK;
-  s.counts = nullptr;
   switch (r0.r.op) {
     case PX_OP_REDUCE: rc = parallax::run_seg_reduce(*s.ctx, r0.r.dtype, b, sp, b); break;
     case PX_OP_SCAN: rc = parallax::run_seg_scan(*s.ctx, r0.r.dtype, b, b, sp); break;
-    case PX_OP_SORT: sp.out = nullptr; rc = parallax::run_seg_sort(*s.ctx, r0.r.dtype, b, sp); break;
-    case PX_OP_FLAGS: {
-      s.counts = pool.allocate(nseg * sizeof(uint64_t));
-      if (!s.counts) return false;
-      if (s.pin_cap < nseg) {
-        if (s.pin_counts) (void)hipHostFree(s.pin_counts);
-        s.pin_counts = nullptr;
-        s.pin_cap = 0;
-        if (hipHostMalloc(reinterpret_cast<void**>(&s.pin_counts), nseg * sizeof(uint64_t),
-                          hipHostMallocDefault) != hipSuccess)
-          return false;
-        s.pin_cap = nseg;
-      }
+    case PX_OP_SORT: rc = parallax::run_seg_sort(*s.ctx, r0.r.dtype, b, sp, s.scratch, scratch_bytes); break;
+    case PX_OP_FLAGS:
       rc = parallax::run_seg_compact(*s.ctx, r0.r.flags, b, b, sp, static_cast<uint64_t*>(s.counts));
       if (rc == PX_EOK &&
           hipMemcpyAsync(s.pin_counts, s.counts, nseg * sizeof(uint64_t), hipMemcpyDeviceToHost,
                          s.ctx->stream()) != hipSuccess)
         rc = PX_EDEVICE;
       break;
-    }
     default: rc = PX_EARG;
   }
-  if (rc != PX_EOK) return false;
-  if (hipEventRecord(s.ev, s.ctx->stream()) != hipSuccess) return false;
+  // Kernels queued before a failure may still run on the request buffers: the batch
+  // completes (with rc) when the slot's event fires, never before.
+  if (hipEventRecord(s.ev, s.ctx->stream()) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s.ctx->stream());
+    return fail_now(rc != PX_EOK ? rc : PX_EDEVICE);
+  }
   s.ids = ids;
   s.busy = true;
-  ++batches;
+  s.rc = rc;
   for (int64_t id : ids) reqs[id].st = St::RUNNING;
-  return true;
+  if (rc != PX_EOK) return kFailed;
+  ++batches;
+  return kLaunched;
 }
 
 void pxs_server::loop() {
@@ -150,43 +257,41 @@ void pxs_server::loop() {
   std::unique_lock<std::mutex> lk(mu);
   while (!stop) {
     bool progress = false;
+    wake = false;
     // 1. completions
     for (auto& s : slots) {
-      if (!s.busy || hipEventQuery(s.ev) != hipSuccess) continue;
-      const double t = now_s();
-      for (size_t i = 0; i < s.ids.size(); ++i) {
-        Req& q = reqs[s.ids[i]];
-        q.st = St::DONE;
-        q.t_done = t;
-        if (s.counts) q.count = s.pin_counts[i];
-        lat_ms.push_back((t - q.t_arr) * 1e3);
-        ++completed;
-      }
-      last_done = t;
-      if (s.counts) pool.deallocate(s.counts);
-      s.counts = nullptr;
-      s.ids.clear();
-      s.busy = false;
+      if (!s.busy || hipEventQuery(s.ev) == hipErrorNotReady) continue;
+      finish_slot(s, now_s());
       progress = true;
     }
-    if (progress) cv_done.notify_all();
     // 2. admission in arrival order
+    bool head_waits = false;
     while (!pending.empty()) {
-      Req& q = reqs[pending.front()];
-      size_t need = 0;
-      if (q.r.op == PX_OP_REDUCE) need = q.es;
-      else if (q.r.op == PX_OP_SCAN || q.r.op == PX_OP_FLAGS) need = q.r.n * q.es;
+      const int64_t id = pending.front();
+      Req& q = reqs[id];
+      const size_t need = out_bytes(q.r, q.es);
       if (need) {
         q.d_out = pool.allocate(need);
-        if (!q.d_out) break;
+        if (!q.d_out) {
+          if (round_pool(need) > attainable()) {
+            // never fits beside the client's inputs: fail it rather than block the queue
+            fail(id, PX_ENOMEM, now_s());
+            pending.pop_front();
+            progress = true;
+            continue;
+          }
+          head_waits = true;
+          break;
+        }
       }
       q.st = St::READY;
-      ready.push_back(pending.front());
+      ready.push_back(id);
       pending.pop_front();
       progress = true;
     }
     peak = std::max(peak, pool.used());
     // 3. dispatch to free slots
+    bool dispatch_waits = false;
     for (auto& s : slots) {
       if (s.busy || ready.empty()) continue;
       const Req& head = reqs[ready.front()];
@@ -196,27 +301,62 @@ void pxs_server::loop() {
         if (static_cast<int>(same) < cfg.max_batch) continue;
       }
       std::vector<int64_t> ids;
-      std::deque<int64_t> rest;
-      for (int64_t id : ready) {
-        if (static_cast<int>(ids.size()) < cfg.max_batch && same_group(head, reqs[id]))
-          ids.push_back(id);
-        else
-          rest.push_back(id);
+      for (int64_t id : ready)
+        if (static_cast<int>(ids.size()) < cfg.max_batch && same_group(head, reqs[id])) ids.push_back(id);
+      int lr = launch(s, ids);
+      if (lr == kNoMemory && !any_busy() && ids.size() > 1) {
+        ids.resize(1);  // the head alone
+        lr = launch(s, ids);
       }
-      if (!launch(s, ids)) {
-        // a failed launch completes its requests with no result (counted as rejected)
-        for (int64_t id : ids) {
-          Req& q = reqs[id];
-          q.st = St::DONE;
-          q.t_done = now_s();
-          ++rejected;
+      if (lr == kNoMemory) {
+        if (any_busy()) break;  // running batches free their scratch when they finish
+        const Req& q = reqs[ids.front()];
+        const char* base = static_cast<const char*>(pool.device_base());
+        const uint64_t lo = static_cast<uint64_t>(static_cast<const char*>(q.r.d_in) - base) / q.es;
+        const size_t sc = q.r.op == PX_OP_SORT ? sort_scratch(q.r.dtype, lo, q.r.n) : 64;
+        if (round_pool(sc) > attainable()) {
+          fail(ids.front(), PX_ENOMEM, now_s());
+          ready.pop_front();
+          progress = true;
+          continue;
         }
-        cv_done.notify_all();
+        dispatch_waits = true;
+        break;
+      }
+      // launched, or failed: take the batch out of the ready queue
+      std::deque<int64_t> rest;
+      size_t k = 0;
+      for (int64_t id : ready) {
+        if (k < ids.size() && ids[k] == id) ++k;
+        else rest.push_back(id);
       }
       ready.swap(rest);
       progress = true;
     }
-    if (!progress) cv_work.wait_for(lk, std::chrono::microseconds(20));
+    const bool now_stalled = (head_waits || dispatch_waits) && !any_busy();
+    if (now_stalled != stalled) progress = true;
+    stalled = now_stalled;
+    if (progress) cv_done.notify_all();
+    if (progress || wake.load()) continue;
+    if (any_busy()) {
+      // batches are on the device: poll their events with the lock released (only this
+      // thread touches the slots), until one fires, a client wakes us, or 100 us pass
+      std::vector<hipEvent_t> evs;
+      for (const auto& s : slots)
+        if (s.busy) evs.push_back(s.ev);
+      lk.unlock();
+      const auto t0 = std::chrono::steady_clock::now();
+      bool fired = false;
+      while (!fired && !wake.load() &&
+             std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(100)) {
+        for (hipEvent_t e : evs)
+          if (hipEventQuery(e) != hipErrorNotReady) fired = true;
+        if (!fired) std::this_thread::yield();
+      }
+      lk.lock();
+    } else {
+      cv_work.wait(lk, [&] { return stop || wake.load(); });
+    }
   }
 }
 
@@ -281,6 +421,10 @@ void* pxs_alloc(pxs_server_t s, size_t bytes) {
   if (!s) return nullptr;
   std::lock_guard<std::mutex> lk(s->mu);
   void* p = s->pool.allocate(bytes);
+  if (p) {
+    s->client_allocs[p] = round_pool(bytes);
+    s->client_bytes += round_pool(bytes);
+  }
   s->peak = std::max(s->peak, s->pool.used());
   return p;
 }
@@ -288,7 +432,13 @@ void* pxs_alloc(pxs_server_t s, size_t bytes) {
 void pxs_free(pxs_server_t s, void* d_ptr) {
   if (!s) return;
   std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->client_allocs.find(d_ptr);
+  if (it == s->client_allocs.end()) return;  // not a pxs_alloc pointer: ignored
+  s->client_bytes -= it->second;
+  s->client_allocs.erase(it);
   s->pool.deallocate(d_ptr);
+  s->stalled = false;
+  s->wake = true;
   s->cv_work.notify_all();
 }
 
@@ -304,10 +454,38 @@ int64_t pxs_submit(pxs_server_t s, const pxs_request* req) {
   const size_t es = pxk::dtype_size(req->dtype);
   std::lock_guard<std::mutex> lk(s->mu);
   if (req->n && (!s->pool.contains(req->d_in) ||
-                 !s->pool.contains(static_cast<char*>(req->d_in) + req->n * es - 1)))
+                 !s->pool.contains(static_cast<char*>(req->d_in) + req->n * es - 1))) {
+    parallax::set_last_error(PX_EARG, "pxs_submit: input outside the server's pool");
     return PX_EARG;
-  if (req->n * es > s->pool.capacity()) {
+  }
+  const char* base = static_cast<const char*>(s->pool.device_base());
+  const uint64_t off = static_cast<uint64_t>(static_cast<const char*>(req->d_in) - base);
+  if (req->n && off % es != 0) {
+    parallax::set_last_error(PX_EARG, "pxs_submit: input not aligned to its element size");
+    return PX_EARG;
+  }
+  // An in-place sort must not share bytes with another request that has not completed
+  // (they could run concurrently on different slots).
+  if (req->n) {
+    const uint64_t a0 = off, a1 = off + req->n * es;
+    for (const auto& kv : s->reqs) {
+      const Req& q = kv.second;
+      if (q.st == St::DONE || q.r.n == 0) continue;
+      if (req->op != PX_OP_SORT && q.r.op != PX_OP_SORT) continue;
+      const uint64_t b0 = static_cast<uint64_t>(static_cast<const char*>(q.r.d_in) - base);
+      const uint64_t b1 = b0 + q.r.n * q.es;
+      if (a0 < b1 && b0 < a1) {
+        parallax::set_last_error(PX_EARG, "pxs_submit: input overlaps a pending in-place sort");
+        return PX_EARG;
+      }
+    }
+  }
+  // outputs (and a sort's scratch) that could never fit beside the client's inputs
+  const size_t need = req->op == PX_OP_SORT ? sort_scratch(req->dtype, off / es, req->n)
+                                             : out_bytes(*req, es);
+  if (round_pool(need) > s->attainable()) {
     ++s->rejected;
+    parallax::set_last_error(PX_ENOMEM, "pxs_submit: the request's output cannot fit in the pool");
     return PX_ENOMEM;
   }
   const int64_t id = s->next_id++;
@@ -319,6 +497,7 @@ int64_t pxs_submit(pxs_server_t s, const pxs_request* req) {
   s->reqs.emplace(id, q);
   s->pending.push_back(id);
   ++s->submitted;
+  s->wake = true;
   s->cv_work.notify_all();
   return id;
 }
@@ -326,15 +505,25 @@ int64_t pxs_submit(pxs_server_t s, const pxs_request* req) {
 int pxs_wait(pxs_server_t s, int64_t id, double timeout_ms) {
   if (!s) return PX_EARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  auto done = [&] {
+  auto it0 = s->reqs.find(id);
+  if (it0 == s->reqs.end()) return PX_EARG;
+  auto settled = [&] {
     auto it = s->reqs.find(id);
-    return it == s->reqs.end() || it->second.st == St::DONE;
+    if (it == s->reqs.end() || it->second.st == St::DONE) return true;
+    return s->stalled && it->second.st != St::RUNNING;
   };
   if (timeout_ms < 0) {
-    s->cv_done.wait(lk, done);
-    return 0;
+    s->cv_done.wait(lk, settled);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), settled)) {
+    return 1;
   }
-  return s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done) ? 0 : 1;
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return PX_EARG;
+  if (it->second.st != St::DONE) {
+    parallax::set_last_error(PX_ENOMEM, "pxs_wait: the request waits for pool memory the client holds");
+    return PX_ENOMEM;
+  }
+  return it->second.status;
 }
 
 int pxs_result(pxs_server_t s, int64_t id, void** d_out, uint64_t* count, double* latency_ms) {
@@ -346,7 +535,7 @@ int pxs_result(pxs_server_t s, int64_t id, void** d_out, uint64_t* count, double
   if (d_out) *d_out = q.d_out ? q.d_out : q.r.d_in;
   if (count) *count = q.count;
   if (latency_ms) *latency_ms = (q.t_done - q.t_arr) * 1e3;
-  return PX_EOK;
+  return q.status;
 }
 
 int pxs_release(pxs_server_t s, int64_t id) {
@@ -356,6 +545,8 @@ int pxs_release(pxs_server_t s, int64_t id) {
   if (it == s->reqs.end() || it->second.st != St::DONE) return PX_EARG;
   if (it->second.d_out) s->pool.deallocate(it->second.d_out);
   s->reqs.erase(it);
+  s->stalled = false;
+  s->wake = true;
   s->cv_work.notify_all();
   return PX_EOK;
 }
@@ -363,8 +554,10 @@ int pxs_release(pxs_server_t s, int64_t id) {
 int pxs_drain(pxs_server_t s) {
   if (!s) return PX_EARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  s->cv_done.wait(lk, [&] { return s->completed + s->rejected >= s->submitted; });
-  return PX_EOK;
+  s->cv_done.wait(lk, [&] { return s->completed + s->failed >= s->submitted || s->stalled; });
+  if (s->completed + s->failed >= s->submitted) return PX_EOK;
+  parallax::set_last_error(PX_ENOMEM, "pxs_drain: requests wait for pool memory the client holds");
+  return PX_ENOMEM;
 }
 
 int pxs_get_stats(pxs_server_t s, pxs_stats* out) {
@@ -375,6 +568,7 @@ int pxs_get_stats(pxs_server_t s, pxs_stats* out) {
   out->completed = s->completed;
   out->batches = s->batches;
   out->rejected = s->rejected;
+  out->failed = s->failed;
   std::vector<double> v = s->lat_ms;
   if (!v.empty()) {
     std::sort(v.begin(), v.end());

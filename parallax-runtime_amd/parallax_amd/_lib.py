@@ -120,7 +120,7 @@ class PxsStats(ctypes.Structure):
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
                 ("first_arrival_s", ctypes.c_double), ("last_completion_s", ctypes.c_double),
                 ("req_per_s", ctypes.c_double), ("pool_bytes", ctypes.c_size_t),
-                ("pool_peak_bytes", ctypes.c_size_t)]
+                ("pool_peak_bytes", ctypes.c_size_t), ("failed", ctypes.c_uint64)]
 
 
 _proto("pxs_config_default", None, ctypes.POINTER(PxsConfig))

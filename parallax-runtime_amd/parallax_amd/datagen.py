@@ -142,3 +142,48 @@ def plain_suite_inputs(seed=1, scale=1.0):
         "sort_f64": with_duplicates(uniform01_f64(seed, n25, stream=5), seed, 1, stream=50),
         "copy_if_i32": full_i32(seed, n28, stream=6),
     }
+
+
+def batched_b1(seed=2, nseg=16384):
+    """B1: nseg float32 segments, lengths log-uniform in [1, 2^18], values normal(0, 1)."""
+    lens = log_uniform_lengths(seed, nseg, 1, 1 << 18, stream=1)
+    off = offsets_from_lengths(lens)
+    return normal_f32(seed, int(off[-1]), stream=2), off
+
+
+def batched_b2(seed=3, nseg=4096):
+    """B2: nseg uint64 key segments, lengths log-uniform in [16, 2^18], keys uniform in
+    [0, 2^20) (about 30% duplicates per segment)."""
+    lens = log_uniform_lengths(seed, nseg, 16, 1 << 18, stream=1)
+    off = offsets_from_lengths(lens)
+    return int_range(seed, int(off[-1]), 0, 1 << 20, np.uint64, stream=2), off
+
+
+def served_requests(seed=4, nreq=400):
+    """The served mixed stream: per request (op, dtype, values, predicate constant).
+    Ops 30% reduce, 30% scan, 25% sort, 15% copy_if (predicate x < c, c = 500 for
+    uint64 and 0 otherwise); sizes log-uniform in [2^10, 2^24]; dtypes uniform over
+    {int32, uint64, float32, float64}; values uniform in [-1000, 1000] ([0, 1000] for
+    uint64)."""
+    from . import consts as C
+    from .dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_U64
+    u = uniform01_f64(seed, 4 * nreq, stream=1)
+    ops = np.array([C.OP_REDUCE, C.OP_SCAN, C.OP_SORT, C.OP_FLAGS])
+    op_of = ops[np.searchsorted(np.cumsum([0.3, 0.3, 0.25, 0.15]), u[:nreq], side="right").clip(0, 3)]
+    dts = [PX_I32, PX_U64, PX_F32, PX_F64]
+    dt_of = [dts[int(x * 4) % 4] for x in u[nreq:2 * nreq]]
+    sizes = np.floor(2.0 ** (10 + 14 * u[2 * nreq:3 * nreq])).astype(np.int64)
+    def one(i):
+        dt, n = dt_of[i], int(sizes[i])
+        if dt == PX_U64:
+            a = int_range(seed, n, 0, 1001, np.uint64, stream=100 + i)
+        elif dt == PX_I32:
+            a = int_range(seed, n, -1000, 1001, np.int32, stream=100 + i)
+        else:
+            a = (uniform01_f64(seed, n, stream=100 + i) * 2000.0 - 1000.0).astype(NP_DTYPES[dt])
+        return (int(op_of[i]), dt, a, 500 if dt == PX_U64 else 0)
+
+    # requests in parallel (numpy releases the GIL); a pool of its own, since large
+    # requests use the chunk pool inside u64()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(nreq)))

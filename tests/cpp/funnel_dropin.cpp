@@ -29,7 +29,34 @@ static void check(bool ok, const char* what) {
   if (!ok) ++failures;
 }
 
-int main() {
+// Large in-place unique through the built-in device_unique key: a sorted 2^28 uint64
+// vector (2 GiB, twice the default arena), checked against std::unique on a copy.
+static int big_unique() {
+  const size_t n = size_t(1) << 28;
+  std::vector<uint64_t> v(n);
+  std::mt19937_64 rng(11);
+  uint64_t x = 0;
+  for (size_t i = 0; i < n; ++i) {
+    x += (rng() % 3 == 0) ? 1 + (rng() & 7) : 0;
+    v[i] = x;
+  }
+  std::vector<uint64_t> ref = v;
+  uint64_t st0[7] = {0}, st1[7] = {0};
+  pxr_stats(st0);
+  auto e = parallax::unique(parallax::par, v.begin(), v.end());
+  pxr_stats(st1);
+  auto e_ref = std::unique(ref.begin(), ref.end());
+  const size_t k = static_cast<size_t>(e - v.begin());
+  const size_t k_ref = static_cast<size_t>(e_ref - ref.begin());
+  check(k == k_ref && std::memcmp(v.data(), ref.data(), k * sizeof(uint64_t)) == 0,
+        "unique 2^28 uint64 in place");
+  std::printf("kept %zu of %zu\n", k, n);
+  std::printf("device_launches %llu\n", static_cast<unsigned long long>(st1[0] - st0[0]));
+  return failures == 0 ? 0 : 1;
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1 && std::strcmp(argv[1], "big") == 0) return big_unique();
   // the descriptor a compiler registrar would emit for device_copy_if<int, IsEven>
   static px_desc flags{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_FLAGS, PX_I32, 0, PX_PRED_MOD_EQ, 0, 0, 2, 0};
   static px_desc scan{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_SCAN, PX_I32, 0, 0, 0, 0, 0, 0};
@@ -43,6 +70,24 @@ int main() {
   parallax_kernel_register((k + ":scan").c_str(), reinterpret_cast<unsigned*>(&scan), PX_DESC_WORDS);
   parallax_kernel_register((k + ":add").c_str(), reinterpret_cast<unsigned*>(&add), PX_DESC_WORDS);
   parallax_kernel_register((k + ":scatter").c_str(), reinterpret_cast<unsigned*>(&scatter), PX_DESC_WORDS);
+
+  // partition and remove_if with registered functors, in place on the caller's vector
+  static px_desc part{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_SCATTER, PX_I32, PX_SCATTER_PARTITION, 0, 0, 0, 0, 0};
+  static px_desc not_even{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_FLAGS, PX_I32, 0, PX_PRED_MOD_EQ, PX_FLAG_NEGATE, 0, 2, 0};
+  const std::string kp(
+      "std::size_t parallax::detail::device_partition(T*, std::size_t, Pred) "
+      "[with T = int; Pred = IsEven; std::size_t = long unsigned int]");
+  const std::string kr(
+      "std::size_t parallax::detail::device_remove_if(T*, std::size_t, Pred) "
+      "[with T = int; Pred = IsEven; std::size_t = long unsigned int]");
+  parallax_kernel_register((kp + ":flags").c_str(), reinterpret_cast<unsigned*>(&flags), PX_DESC_WORDS);
+  parallax_kernel_register((kp + ":scan").c_str(), reinterpret_cast<unsigned*>(&scan), PX_DESC_WORDS);
+  parallax_kernel_register((kp + ":add").c_str(), reinterpret_cast<unsigned*>(&add), PX_DESC_WORDS);
+  parallax_kernel_register((kp + ":scatter").c_str(), reinterpret_cast<unsigned*>(&part), PX_DESC_WORDS);
+  parallax_kernel_register((kr + ":flags").c_str(), reinterpret_cast<unsigned*>(&not_even), PX_DESC_WORDS);
+  parallax_kernel_register((kr + ":scan").c_str(), reinterpret_cast<unsigned*>(&scan), PX_DESC_WORDS);
+  parallax_kernel_register((kr + ":add").c_str(), reinterpret_cast<unsigned*>(&add), PX_DESC_WORDS);
+  parallax_kernel_register((kr + ":scatter").c_str(), reinterpret_cast<unsigned*>(&scatter), PX_DESC_WORDS);
 
   // count_if and transform_reduce with registered functors: one fused device pass each
   static px_desc pred{PX_DESC_MAGIC, PX_DESC_VERSION, PX_OP_TRANSFORM, PX_I32, PX_I32, PX_MAP_PRED, 0, PX_PRED_MOD_EQ, 2, 0};
@@ -99,6 +144,17 @@ int main() {
   std::copy_if(vi.begin(), vi.end(), std::back_inserter(out_ref), IsEven{});
   check(static_cast<size_t>(oe - out.begin()) == out_ref.size() &&
             std::equal(out_ref.begin(), out_ref.end(), out.begin()), "copy_if IsEven");
+  // partition: the same bytes as the host loop's std::partition (GPU or not)
+  std::vector<int> pv = vi, pv_ref = vi;
+  auto pm = parallax::partition(parallax::par, pv.begin(), pv.end(), IsEven{});
+  auto pm_ref = std::partition(pv_ref.begin(), pv_ref.end(), IsEven{});
+  check((pm - pv.begin()) == (pm_ref - pv_ref.begin()) && pv == pv_ref, "partition IsEven (std order)");
+  // remove_if in place
+  std::vector<int> rv = vi, rv_ref = vi;
+  auto rm = parallax::remove_if(parallax::par, rv.begin(), rv.end(), IsEven{});
+  auto rm_ref = std::remove_if(rv_ref.begin(), rv_ref.end(), IsEven{});
+  check((rm - rv.begin()) == (rm_ref - rv_ref.begin()) && std::equal(rv.begin(), rm, rv_ref.begin()),
+        "remove_if IsEven");
   // registered count_if / transform_reduce (fused map + sum on the device)
   const long ce = parallax::count_if(parallax::par, vi.begin(), vi.end(), IsEven{});
   check(ce == std::count_if(vi.begin(), vi.end(), IsEven{}), "count_if IsEven");

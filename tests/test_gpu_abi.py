@@ -155,15 +155,23 @@ def test_abi_partition_unique_remove(gpu_ctx):
     kp = load(make_desc(C.OP_SCATTER, PX_I64, aux=C.SCATTER_PARTITION))
     out = np.zeros_like(a)
     nt = lib.parallax_copy_if(kf, ks, ka, kp, a.ctypes.data, out.ctypes.data, a.size, 8, 0)
-    sref, sk = pxo.partition(make_desc(C.OP_FLAGS, PX_I64, fn=C.PRED_LT, c0=7), a, stable=True)
-    assert nt == sk and np.array_equal(out, sref)
+    ref, rk = pxo.partition(make_desc(C.OP_FLAGS, PX_I64, fn=C.PRED_LT, c0=7), a)
+    assert nt == rk and out.tobytes() == ref.tobytes()  # std::partition's order
+    w = a.copy()  # in place (host memory staged once), as the partition funnel calls it
+    nt = lib.parallax_copy_if(kf, ks, ka, kp, w.ctypes.data, w.ctypes.data, a.size, 8, 0)
+    assert nt == rk and w.tobytes() == ref.tobytes()
     out[:] = 0
     k = lib.parallax_copy_if(kfn, ks, ka, kc, a.ctypes.data, out.ctypes.data, a.size, 8, 0)  # remove_if
     assert np.array_equal(out[:k], a[a >= 7])
+    w = a.copy()
+    k = lib.parallax_copy_if(kfn, ks, ka, kc, w.ctypes.data, w.ctypes.data, a.size, 8, 0)  # in place
+    assert np.array_equal(w[:k], a[a >= 7]) and np.array_equal(w[k:], a[k:])
     s = np.sort(a)
     ku = load(make_desc(C.OP_FLAGS, PX_I64, fn=C.PRED_UNIQUE))
     k = lib.parallax_copy_if(ku, ks, ka, kc, s.ctypes.data, out.ctypes.data, a.size, 8, 0)
     assert np.array_equal(out[:k], np.unique(a))
+    k = lib.parallax_copy_if(ku, ks, ka, kc, s.ctypes.data, s.ctypes.data, a.size, 8, 0)  # in place
+    assert np.array_equal(s[:k], np.unique(a))
 
 
 def test_abi_map_skeletons(gpu_ctx):

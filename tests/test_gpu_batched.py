@@ -584,3 +584,111 @@ def test_served_fuses_queued_requests():
     for rid, _ in ids_b + ids_s:
         lib.pxs_release(server, rid)
     lib.pxs_destroy(server)
+
+
+def test_served_output_that_never_fits_is_refused():
+    # the pool holds the input and little else: a scan output of the same size can never
+    # fit beside it, so pxs_submit refuses it at once (nothing blocks the queue)
+    n = 1 << 20
+    a = np.arange(n, dtype=np.uint32)
+    server = make_server(n * 4 + (1 << 16))
+    p = put(server, a)
+    fl = make_desc(C.OP_FLAGS, PX_U32, fn=C.PRED_LT, c0=10)
+    rid = lib.pxs_submit(server, ctypes.byref(PxsRequest(C.OP_SCAN, PX_U32, n, p, fl)))
+    assert rid == C.E_NOMEM
+    # a reduce (one element of output) still runs
+    rid = lib.pxs_submit(server, ctypes.byref(PxsRequest(C.OP_REDUCE, PX_U32, n, p, fl)))
+    assert rid >= 0
+    check_served(server, rid, C.OP_REDUCE, PX_U32, a)
+    st = PxsStats()
+    lib.pxs_get_stats(server, ctypes.byref(st))
+    assert st.rejected == 1 and st.failed == 0
+    lib.pxs_destroy(server)
+
+
+def test_served_drain_before_release_does_not_block_forever():
+    # room for the inputs plus about three outputs; a client that drains before releasing
+    # gets PX_ENOMEM back (not a hang), releases what completed, and drains again
+    n = 1 << 20
+    arrays = [np.arange(n, dtype=np.uint32) * 3 + i for i in range(12)]
+    server = make_server(12 * n * 4 + 3 * n * 4 + (1 << 16), slots=2)
+    ptrs = [put(server, a) for a in arrays]
+    ids = [submit(server, C.OP_SCAN, PX_U32, a, p)[0] for a, p in zip(arrays, ptrs)]
+    done = set()
+    stalls = 0
+    for _ in range(100):
+        rc = lib.pxs_drain(server)
+        assert rc in (0, C.E_NOMEM)
+        for a, rid in zip(arrays, ids):
+            if rid in done:
+                continue
+            w = lib.pxs_wait(server, rid, 0.0)
+            if w == 0:
+                check_served(server, rid, C.OP_SCAN, PX_U32, a)
+                check(lib.pxs_release(server, rid), "release")
+                done.add(rid)
+        if rc == 0 and len(done) == len(ids):
+            break
+        stalls += rc != 0
+    assert len(done) == len(ids) and stalls >= 1
+    lib.pxs_destroy(server)
+
+
+def test_served_batch_scratch_wait_reports_enomem_then_runs():
+    # inputs and the output fill the pool exactly: the copy_if batch's count words do not
+    # fit, nothing runs, so pxs_wait returns PX_ENOMEM; freeing a client buffer lets it run
+    n = 1 << 16
+    a = np.random.default_rng(3).integers(-100, 100, n).astype(np.int32)
+    server = make_server(2 * n * 4 + 256)
+    p = put(server, a)
+    spare = lib.pxs_alloc(server, 256)
+    assert spare
+    fl = make_desc(C.OP_FLAGS, PX_I32, fn=C.PRED_LT, c0=0)
+    rid = lib.pxs_submit(server, ctypes.byref(PxsRequest(C.OP_FLAGS, PX_I32, n, p, fl)))
+    assert rid >= 0
+    assert lib.pxs_wait(server, rid, 5000.0) == C.E_NOMEM
+    lib.pxs_free(server, spare)
+    assert lib.pxs_wait(server, rid, 5000.0) == 0
+    check_served(server, rid, C.OP_FLAGS, PX_I32, a, fl)
+    lib.pxs_destroy(server)
+
+
+def test_served_rejects_misaligned_and_overlapping_inputs():
+    server = make_server(1 << 30)
+    big = [np.random.default_rng(i).random(1 << 23) for i in range(4)]
+    pb = [put(server, a) for a in big]
+    x = np.arange(1 << 16, dtype=np.uint32)
+    px = put(server, x)
+    fl = make_desc(C.OP_FLAGS, PX_U32, fn=C.PRED_LT, c0=10)
+    assert lib.pxs_submit(server, ctypes.byref(PxsRequest(C.OP_REDUCE, PX_U32, 100, px + 2, fl))) == C.E_ARG
+    # four long sorts hold the slots; a sort of x queues behind them, so a reduce of x
+    # submitted now would share its bytes with a pending in-place sort
+    ids = [submit(server, C.OP_SORT, PX_F64, a, p)[0] for a, p in zip(big, pb)]
+    rs = submit(server, C.OP_SORT, PX_U32, x, px)[0]
+    assert lib.pxs_submit(server, ctypes.byref(PxsRequest(C.OP_REDUCE, PX_U32, x.size, px, fl))) == C.E_ARG
+    check(lib.pxs_drain(server), "drain")
+    check_served(server, rs, C.OP_SORT, PX_U32, x)
+    for rid in ids + [rs]:
+        lib.pxs_release(server, rid)
+    # once the sort is done, the same reduce is accepted
+    rid = lib.pxs_submit(server, ctypes.byref(PxsRequest(C.OP_REDUCE, PX_U32, x.size, px, fl)))
+    assert rid >= 0
+    check_served(server, rid, C.OP_REDUCE, PX_U32, np.sort(x))
+    lib.pxs_destroy(server)
+
+
+def test_served_sort_deep_in_the_pool_uses_little_scratch():
+    # a sort of 2^22 u64 keys 6 GiB into an 8 GiB pool: its radix scratch (about twice
+    # its own bytes) comes from the pool and does not depend on where the keys sit
+    server = make_server(8 << 30)
+    filler = lib.pxs_alloc(server, 6 << 30)
+    assert filler
+    a = np.random.default_rng(5).integers(0, 1 << 63, 1 << 22, dtype=np.uint64)
+    p = put(server, a)
+    rid = submit(server, C.OP_SORT, PX_U64, a, p)[0]
+    check_served(server, rid, C.OP_SORT, PX_U64, a)
+    st = PxsStats()
+    lib.pxs_get_stats(server, ctypes.byref(st))
+    assert st.pool_peak_bytes <= (6 << 30) + 3 * a.nbytes + (1 << 22), st.pool_peak_bytes
+    lib.pxs_release(server, rid)
+    lib.pxs_destroy(server)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import pxo
-from parallax_amd import lib, check, make_desc, consts as C
+from parallax_amd import lib, check, make_desc, descriptor_words, consts as C
 from parallax_amd.device import DeviceBuffer
 from parallax_amd.dtypes import NP_DTYPES, PX_F32, PX_F64, PX_I32, PX_I64, PX_U32, PX_U64, dtype_code
 
@@ -332,22 +332,96 @@ def test_unique(gpu_ctx, dt, n):
 
 
 @pytest.mark.parametrize("dt", ALL)
-@pytest.mark.parametrize("n", [1, 4097, 300_001])
-def test_partition_stable(gpu_ctx, dt, n):
+@pytest.mark.parametrize("kind", ["all", "half", "unique_most", "unique_few"])
+def test_compaction_in_place(gpu_ctx, dt, kind):
+    """input == output (remove_if / unique funnels): every tile writes below its own end,
+    after the tiles before it have read theirs; many tiles, misaligned start."""
+    n = 1_000_003
+    if kind.startswith("unique"):
+        hi = 1 << 40 if kind == "unique_most" else 300
+        a = np.sort(gen(dt, n, 950 + dt, *((None, None) if dt in FLOATS else (0, min(hi, 1 << 30)))))
+        if dt in FLOATS and kind == "unique_few":
+            a = np.round(a, 1).astype(a.dtype)
+        d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_UNIQUE)
+        ref = pxo.unique(a)
+    else:
+        a = gen(dt, n, 960 + dt)
+        d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_TRUE) if kind == "all" else \
+            make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=0 if dt not in (PX_U32, PX_U64) else 1 << 29)
+        ref = pxo.copy_if(d, a)
+    b, p = upload(a, 3)
+    cnt = DeviceBuffer(8)
+    check(lib.pxd_copy_if(gpu_ctx.h, ctypes.byref(d), p, p, n, cnt.ptr), "copy_if in place")
+    gpu_ctx.sync()
+    k = int(cnt.download(np.uint64, 1)[0])
+    got = b.download(a.dtype, n, 3 * a.itemsize)
+    assert k == ref.size and got[:k].tobytes() == ref.tobytes()
+    assert got[k:].tobytes() == a[k:].tobytes()  # nothing written past the kept count
+
+
+PART_SIZES = [1, 2, 3, 64, 4097, 65_536, 300_001, 1_000_003]
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", PART_SIZES)
+@pytest.mark.parametrize("in_place", [False, True])
+def test_partition_std_order(gpu_ctx, dt, n, in_place):
+    """Bit-exact with std::partition (the partition funnel's fallback, reference
+    stdpar.hpp:369), out of place and in place, misaligned input."""
     a = gen(dt, n, 800 + n + dt)
     d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=0 if dt not in (PX_U32, PX_U64) else 1 << 29)
-    ib, ip = upload(a)
-    ob, op = upload(np.zeros_like(a))
+    ib, ip = upload(a, 1)
+    if in_place:
+        ob, op = ib, ip
+    else:
+        ob, op = upload(np.zeros_like(a), 2)
     nt = DeviceBuffer(8)
     check(lib.pxd_partition(gpu_ctx.h, ctypes.byref(d), ip, op, n, nt.ptr), "partition")
     gpu_ctx.sync()
     k = int(nt.download(np.uint64, 1)[0])
-    sref, sk = pxo.partition(d, a, stable=True)
     ref, rk = pxo.partition(d, a)
-    got = ob.download(a.dtype, n)
-    assert k == sk == rk
-    assert got.tobytes() == sref.tobytes()  # the GPU's order is the stable one
-    assert sorted(got[:k].tolist()) == sorted(ref[:k].tolist())  # same sides as std::partition
+    got = ob.download(a.dtype, n, (1 if in_place else 2) * a.itemsize)
+    assert k == rk
+    assert got.tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("dt", [PX_I32, PX_U64, PX_F64])
+@pytest.mark.parametrize("frac", [0.0, 0.01, 0.5, 0.99, 1.0])
+def test_partition_std_order_split_fractions(gpu_ctx, dt, frac):
+    # every element kept, none kept and lopsided splits (m = 0 and tiny m)
+    n = 200_003
+    rng = np.random.default_rng(int(frac * 100) + dt)
+    flags = rng.random(n) < frac
+    a = np.where(flags, 1, -1).astype(NP_DTYPES[dt]) if dt != PX_U64 else np.where(flags, 2, 1).astype(np.uint64)
+    a = (a * (np.arange(n) % 1000 + 1)).astype(a.dtype)
+    d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=0 if dt != PX_U64 else 1000)
+    b, p = upload(a)
+    nt = DeviceBuffer(8)
+    check(lib.pxd_partition(gpu_ctx.h, ctypes.byref(d), p, p, n, nt.ptr), "partition")
+    gpu_ctx.sync()
+    ref, rk = pxo.partition(d, a)
+    assert int(nt.download(np.uint64, 1)[0]) == rk
+    assert b.download(a.dtype, n).tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("dt", ALL)
+@pytest.mark.parametrize("n", [1, 4097, 300_001])
+@pytest.mark.parametrize("in_place", [False, True])
+def test_stable_partition_mode(gpu_ctx, dt, n, in_place):
+    """Scatter mode PX_SCATTER_STABLE_PARTITION of the C ABI: std::stable_partition."""
+    a = gen(dt, n, 900 + n + dt)
+    d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=0 if dt not in (PX_U32, PX_U64) else 1 << 29)
+    kf = lib.parallax_kernel_load(*descriptor_words(d))
+    ks = lib.parallax_kernel_load(*descriptor_words(make_desc(C.OP_SCAN, dt)))
+    ka = lib.parallax_kernel_load(*descriptor_words(make_desc(C.OP_SCAN_ADD, dt)))
+    kx = lib.parallax_kernel_load(*descriptor_words(make_desc(C.OP_SCATTER, dt, aux=C.SCATTER_STABLE_PARTITION)))
+    assert kf and ks and ka and kx
+    ib, ip = upload(a)
+    ob, op = (ib, ip) if in_place else upload(np.zeros_like(a))
+    k = lib.parallax_copy_if(kf, ks, ka, kx, ip, op, n, a.itemsize, int(dt in FLOATS))
+    sref, sk = pxo.partition(d, a, stable=True)
+    assert k == sk
+    assert ob.download(a.dtype, n).tobytes() == sref.tobytes()
 
 
 MAPS = [(C.MAP_IDENTITY, 0, 0), (C.MAP_FILL, 9, 0), (C.MAP_ADD, 3, 0), (C.MAP_MUL, -2, 0),

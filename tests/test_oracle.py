@@ -168,6 +168,70 @@ def test_partition_point_and_sides():
     assert np.array_equal(sout[:sk], a[a < 400]) and np.array_equal(sout[sk:], a[a >= 400])
 
 
+def _two_pointer_loop(a, flags):
+    """libstdc++'s std::partition for bidirectional iterators (GCC's bits/stl_algo.h
+    __partition), as a plain loop: advance first over trues, retreat last over falses,
+    swap, repeat."""
+    out = list(a)
+    f = list(flags)
+    first, last = 0, len(out)
+    while True:
+        while first != last and f[first]:
+            first += 1
+        if first == last:
+            return out, first
+        last -= 1
+        while first != last and not f[last]:
+            last -= 1
+        if first == last:
+            return out, first
+        out[first], out[last] = out[last], out[first]
+        f[first], f[last] = f[last], f[first]
+        first += 1
+
+
+def _two_pointer_closed_form(a, flags):
+    """The same order in closed form (what the GPU computes): with p = count of trues,
+    the i-th false of [0, p) exchanges with the i-th true of [p, n) counted from the end."""
+    out = a.copy()
+    p = int(flags.sum())
+    left = np.nonzero(~flags[:p])[0]
+    right = (np.nonzero(flags[p:])[0] + p)[::-1]
+    out[left], out[right] = a[right], a[left]
+    return out, p
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 17, 64, 1000])
+def test_partition_two_pointer_loop_equals_closed_form(n):
+    rng = np.random.default_rng(n)
+    for frac in (0.0, 0.3, 0.5, 0.9, 1.0):
+        a = rng.integers(0, 1 << 20, n).astype(np.int64)
+        fl = rng.random(n) < frac
+        lo, lp = _two_pointer_loop(a, fl)
+        co, cp = _two_pointer_closed_form(a, fl)
+        assert lp == cp and np.array_equal(np.array(lo, dtype=np.int64), co)
+
+
+@pytest.mark.parametrize("dt", [PX_I32, PX_U32, PX_I64, PX_U64, PX_F32, PX_F64])
+@pytest.mark.parametrize("n", [1, 2, 7, 4097, 100_003])
+def test_partition_oracle_is_the_two_pointer_order(dt, n):
+    """pxo.partition (std::partition of this libstdc++) against the restated algorithm."""
+    rng = np.random.default_rng(1000 + n + dt)
+    npd = NP_DTYPES[dt]
+    a = (rng.standard_normal(n) * 100).astype(npd) if dt in (PX_F32, PX_F64) else \
+        rng.integers(0, 1 << 30, n).astype(npd)
+    c0 = 0 if dt in (PX_F32, PX_F64, PX_I32, PX_I64) else 1 << 29
+    if dt in (PX_I32, PX_I64):
+        a = (a.astype(np.int64) - (1 << 29)).astype(npd)
+    d = make_desc(C.OP_FLAGS, dt, fn=C.PRED_GT, c0=c0)
+    got, k = pxo.partition(d, a)
+    want, p = _two_pointer_closed_form(a, a > npd.type(c0))
+    assert k == p and got.tobytes() == want.tobytes()
+    if n <= 4097:
+        lo, lp = _two_pointer_loop(a, a > npd.type(c0))
+        assert lp == k and got.tobytes() == np.array(lo, dtype=npd).tobytes()
+
+
 def test_unique_keeps_first_of_runs():
     a = np.array([1, 1, 2, 2, 2, 3, 1, 1, 4], dtype=np.int32)
     assert pxo.unique(a).tolist() == [1, 2, 3, 1, 4]
@@ -211,6 +275,23 @@ def test_segmented_equal_per_segment_plain():
         assert np.array_equal(s[off[i]:off[i + 1]], pxo.inclusive_scan(seg))
         assert np.array_equal(so[off[i]:off[i + 1]], np.sort(seg))
         assert np.array_equal(uq[uo[i]:uo[i + 1]], pxo.unique(np.sort(seg)))
+
+
+def test_seg_scan_exact_restarts_per_segment():
+    # float segments: per-segment values equal the plain exact scan of the segment, and
+    # integer-valued floats give the exact integer prefix sums (a carried-over prefix
+    # from the previous segment would show)
+    rng = np.random.default_rng(9)
+    lens = [3, 0, 1, 1000, 2]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    a = rng.integers(-50, 50, int(off[-1])).astype(np.float32)
+    val, ab = pxo.seg_scan_exact(a, off)
+    for i in range(len(lens)):
+        seg = a[off[i]:off[i + 1]]
+        v, b = pxo.scan_exact(seg)
+        assert np.array_equal(val[off[i]:off[i + 1]], v) and np.array_equal(ab[off[i]:off[i + 1]], b)
+        assert np.array_equal(val[off[i]:off[i + 1]], np.cumsum(seg.astype(np.int64)).astype(np.float64))
+        assert np.array_equal(ab[off[i]:off[i + 1]], np.cumsum(np.abs(seg.astype(np.int64))).astype(np.float64))
 
 
 # ---- search (find / mismatch / argminmax): numpy routines and brute force
