@@ -92,6 +92,15 @@ int pxd_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, void* d_out
 int pxd_transform_reduce(pxd_ctx_t ctx, const px_desc* map, const void* d_in, size_t n,
                          void* d_out);
 int pxd_inclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n);
+/* pxd_inclusive_scan without any allocation: NO COUNTERPART IN THE REFERENCE.  For
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): no context, no hipMalloc / hipFree; the scan's
+ * scratch (about 8 MiB) is a device array of the library that each GPU holds until the
+ * process ends.  Same kernel, tiling and results as pxd_inclusive_scan.  Runs on
+ * hip_stream (NULL: the null stream) of the current device and returns without waiting.
+ * At most 2^31 elements (PX_EARG beyond).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams or threads) race. */
+int pxd_inclusive_scan_noalloc(int dtype, const void* d_in, void* d_out, size_t n, void* hip_stream);
 /* h_init: host bytes of one element (NULL means 0). */
 int pxd_exclusive_scan(pxd_ctx_t ctx, int dtype, const void* d_in, void* d_out, size_t n,
                        const void* h_init);

@@ -317,6 +317,10 @@ int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n,
   return dev_err(pxk::scan(dtype, in, out, g, exclusive, init_host, c.scratch(), c.stream()), "scan");
 }
 
+pxk::SegGeom scan_geometry(int dtype, const void* in, uint64_t n) {
+  return plain_geom(n, dtype, plain_tile(PX_OP_SCAN, dtype), in);
+}
+
 int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::LaunchHook& hook) {
   if (!valid_dtype(dtype) || (n && !data)) return PX_ERR_ARG;
   if (n <= 1) return PX_OK;

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "dev_tiles.hpp"
 #include "kernels.hpp"
 #include "parallax/descriptor.h"
 #include "parallax/hip_backend.hpp"
@@ -113,6 +114,8 @@ int run_reduce(StreamContext& c, int dtype, const void* in, uint64_t n, void* ou
 int run_map_reduce(StreamContext& c, const px_desc& map, const void* in, uint64_t n, void* out_dev);
 int run_scan(StreamContext& c, int dtype, const void* in, void* out, uint64_t n, bool exclusive,
              const void* init_host);
+// Tile geometry of a plain scan of n elements at `in` (the shape run_scan uses).
+pxk::SegGeom scan_geometry(int dtype, const void* in, uint64_t n);
 int run_sort(StreamContext& c, int dtype, void* data, uint64_t n, const pxk::LaunchHook& hook = {});
 // copy_if / remove_if / unique: count to *count_dev (device u64)
 int run_compact(StreamContext& c, const px_desc& flags, const void* in, void* out, uint64_t n,

@@ -82,6 +82,7 @@ _proto("pxd_timer_end", _i, _vp, ctypes.POINTER(ctypes.c_double))
 _proto("pxd_reduce", _i, _vp, _i, _vp, _sz, _vp)
 _proto("pxd_transform_reduce", _i, _vp, _dp, _vp, _sz, _vp)
 _proto("pxd_inclusive_scan", _i, _vp, _i, _vp, _vp, _sz)
+_proto("pxd_inclusive_scan_noalloc", _i, _i, _vp, _vp, _sz, _vp)
 _proto("pxd_exclusive_scan", _i, _vp, _i, _vp, _vp, _sz, _vp)
 _proto("pxd_sort", _i, _vp, _i, _vp, _sz)
 _proto("pxd_copy_if", _i, _vp, _dp, _vp, _vp, _sz, _vp)
