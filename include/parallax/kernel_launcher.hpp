@@ -26,6 +26,8 @@
 
 namespace parallax {
 
+class HostStager;
+
 class StreamContext;
 
 class KernelLauncher {
@@ -89,7 +91,8 @@ class KernelLauncher {
 
  private:
   struct Operand;
-  bool resolve(void* p, size_t bytes, bool read, Operand& op);
+  // slot: which staging buffer a host operand uses (0 first operand, 1 second)
+  bool resolve(void* p, size_t bytes, bool read, Operand& op, int slot = 0);
   bool finish(Operand& op, bool write);
   bool copy_out(void* dst, const void* dev_src, size_t bytes);
   void release(Operand& op);
@@ -114,6 +117,7 @@ class KernelLauncher {
   bool load_code(const std::string& name, const px_desc& d, const uint32_t* words, size_t nwords);
   bool run_code(const std::string& name, const void* in, void* out, uint64_t count,
                 const void* captures, size_t capture_size, bool transform);
+  std::unique_ptr<HostStager> stager_;  // pageable host operands: pinned chunks, kept buffers
   bool prof_on_ = false;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   std::unordered_map<std::string, std::pair<double, uint64_t>> prof_;

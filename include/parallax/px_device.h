@@ -82,6 +82,10 @@ int pxd_timer_begin(pxd_ctx_t ctx);
 int pxd_timer_end(pxd_ctx_t ctx, double* ms);
 
 /* ---- plain primitives (device pointers) */
+/* Device-to-device copy of `bytes` on the context's stream: the streaming-copy kernel
+ * (16-byte non-temporal loads and stores), i.e. the read-and-write bandwidth that the
+ * look-back kernels are compared with.  pxd_memcpy_d2d runs the same kernel. */
+int pxd_copy(pxd_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes);
 /* *d_out = sum of d_in[0..n) in the element type (one element written). */
 int pxd_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, void* d_out);
 /* Σ map(x) in one pass: `map` is a PX_OP_TRANSFORM descriptor (input dtype, output dtype

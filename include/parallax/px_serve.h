@@ -71,6 +71,9 @@ void pxs_free(pxs_server_t s, void* d_ptr);
  * PX_ENOMEM when its output (a sort: its scratch) could never fit in the pool beside the
  * client's inputs.  The arrival time is the time of this call. */
 int64_t pxs_submit(pxs_server_t s, const pxs_request* req);
+/* Queues n requests that arrive together (one lock, one wake-up of the server): ids[i]
+ * receives what pxs_submit would return for reqs[i].  Returns the number refused. */
+int pxs_submit_batch(pxs_server_t s, const pxs_request* reqs, size_t n, int64_t* ids);
 /* Wait for completion (timeout_ms < 0: no limit).  Returns 0 when the request completed,
  * 1 on time-out, the request's negative PX_E* code when it failed, and PX_ENOMEM when it
  * waits for pool memory that only the client can free (nothing is running and outputs

@@ -17,9 +17,11 @@
  *     device scratch and copied back: results always reach the caller's pointer).
  *   - Every primitive is synchronous, as in the reference: outputs are host-visible
  *     (or device-visible for device pointers) when the call returns.
- *   - Void entry points cannot report failure; on an internal device failure the engine
- *     recomputes the result on the host (the dtype is known from the descriptor) rather
- *     than leave garbage in the output.
+ *   - Void entry points cannot return an error code; on a failure the engine prints one
+ *     "[parallax]" line to stderr and records it for pxr_last_error() (px_device.h).  It
+ *     never recomputes on the host: once a call reaches the runtime it runs on the GPU
+ *     or fails loudly (a lookup miss, before the call, is what sends a funnel to its
+ *     CPU loop).
  */
 #ifndef PARALLAX_RUNTIME_H
 #define PARALLAX_RUNTIME_H

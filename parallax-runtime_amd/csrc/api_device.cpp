@@ -2,6 +2,7 @@
 // entry points of parallax/px_device.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -127,7 +128,13 @@ int pxd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost, "memcpy d2h");
 }
 int pxd_memcpy_d2d(void* dst, const void* src, size_t bytes) {
-  return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice, "memcpy d2d");
+  // the streaming-copy kernel on the null stream, waited for
+  parallax::HipBackend* be = parallax::global_backend();
+  if (!be) return nodev();
+  const uint32_t mb = static_cast<uint32_t>(std::max(1, be->capabilities().compute_units)) * 8u;
+  const int rc = hip_rc(pxk::copy_bytes(dst, src, bytes, mb, nullptr), "memcpy d2d");
+  if (rc != PX_EOK) return rc;
+  return hip_rc(hipStreamSynchronize(nullptr), "memcpy d2d");
 }
 int pxd_memset(void* dst, int value, size_t bytes) {
   if (!parallax::global_backend()) return nodev();
@@ -196,6 +203,14 @@ int pxd_timer_end(pxd_ctx_t ctx, double* ms) {
 int pxd_reduce(pxd_ctx_t ctx, int dtype, const void* d_in, size_t n, void* d_out) {
   PXD_GUARD();
   return parallax::run_reduce(*ctx->c, dtype, d_in, n, d_out);
+}
+
+int pxd_copy(pxd_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes) {
+  PXD_GUARD();
+  if (bytes && (!d_dst || !d_src)) return PX_EARG;
+  return hip_rc(pxk::copy_bytes(d_dst, d_src, bytes, static_cast<uint32_t>(ctx->c->cu_count()) * 8u,
+                                ctx->c->stream()),
+                "copy");
 }
 
 int pxd_transform_reduce(pxd_ctx_t ctx, const px_desc* map, const void* d_in, size_t n,

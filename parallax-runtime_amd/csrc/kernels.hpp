@@ -66,6 +66,8 @@ struct Tune {
   int seg_hist_bpc = 0;       // segmented radix histogram blocks per CU (0: default, 4)
   uint64_t seg_radix_min = 0;  // segments longer than this use the segmented radix sort (0: default, 1,024 / 512 keys)
   int seg_radix_block = 0;     // segmented radix digit-pass block, 512 or 1024 (0: chosen per call)
+  int copy_bpc = 0;            // streaming copy: 256-thread blocks per CU (0: default, 1)
+  int copy_unroll = 0;         // ... 16-byte loads in flight per thread, 1/2/4/8 (0: default, 4)
 };
 Tune& tune();
 // Compiled tile shapes of the look-back kernels: op PX_OP_SCAN (scans) or PX_OP_FLAGS
@@ -270,6 +272,10 @@ hipError_t argminmax(int dtype, const void* in, uint64_t n, bool want_max, bool 
                      uint64_t* out_dev, const Scratch& s, hipStream_t stream);
 hipError_t mismatch(int dtype, const void* a, const void* b, uint64_t n, uint64_t* out_dev,
                     const Scratch& s, hipStream_t stream);
+
+// ---- streaming device-to-device copy (k_copy.hip): 16-byte non-temporal vectors when
+// src and dst share their offset within a 16-byte line, else 4-byte words or bytes
+hipError_t copy_bytes(void* dst, const void* src, uint64_t bytes, uint32_t max_blocks, hipStream_t stream);
 
 // ---- fill kernel (graph-safe memset replacement)
 hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);

@@ -12,6 +12,7 @@
 
 #include "engine.hpp"
 #include "parallax/kernel_launcher.hpp"
+#include "staging.hpp"
 
 namespace parallax {
 
@@ -58,12 +59,14 @@ KernelLauncher::KernelLauncher(HipBackend* backend, UnifiedArena* arena)
     : backend_(backend), arena_(arena) {
   if (!backend_ || !backend_->valid()) return;
   ctx_ = std::make_unique<StreamContext>(backend_, backend_->stream());
+  stager_ = std::make_unique<HostStager>();
   if (hipMalloc(reinterpret_cast<void**>(&dev_word_), 64) != hipSuccess) dev_word_ = nullptr;
   if (hipMalloc(&dev_value_, 64) != hipSuccess) dev_value_ = nullptr;
 }
 
 KernelLauncher::~KernelLauncher() {
   if (ctx_) (void)hipStreamSynchronize(ctx_->stream());
+  stager_.reset();
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
@@ -198,7 +201,7 @@ const px_desc* KernelLauncher::find(const std::string& name, const char* who) co
   return d;
 }
 
-bool KernelLauncher::resolve(void* p, size_t bytes, bool read, Operand& op) {
+bool KernelLauncher::resolve(void* p, size_t bytes, bool read, Operand& op, int slot) {
   op = Operand{};
   op.user = p;
   op.bytes = bytes;
@@ -220,26 +223,22 @@ bool KernelLauncher::resolve(void* p, size_t bytes, bool read, Operand& op) {
     op.dev = p;
     return true;
   }
+  // pageable host memory: a device staging buffer kept between calls, filled through
+  // the pinned-chunk pipeline on the launcher's stream (the kernel is queued after it)
   op.kind = Operand::STAGED;
-  if (hipMalloc(&op.dev, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    op.dev = nullptr;
+  op.dev = stager_->device_buffer(slot, bytes);
+  if (!op.dev) {
+    set_last_error(PX_ERR_NOMEM, "staging buffer for a host operand");
     return false;
   }
-  // the copy has landed before the kernel that reads it is queued on the launcher's
-  // (non-blocking) stream: hipMemcpy from pageable memory may return once the data is
-  // staged, so the null stream is waited for too
-  if (read && (hipMemcpy(op.dev, p, bytes, hipMemcpyHostToDevice) != hipSuccess ||
-               hipStreamSynchronize(nullptr) != hipSuccess))
-    return false;
-  return true;
+  return !read || stager_->h2d(op.dev, p, bytes, ctx_->stream());
 }
 
 bool KernelLauncher::finish(Operand& op, bool write) {
   if (wait_stream(ctx_->stream()) != hipSuccess) return false;
   if (!write || op.bytes == 0) return true;
   if (op.kind == Operand::ARENA) return arena_->invalidate_range(op.user, op.bytes);
-  if (op.kind == Operand::STAGED) return copy_out(op.user, op.dev, op.bytes);
+  if (op.kind == Operand::STAGED) return stager_->d2h(op.user, op.dev, op.bytes, ctx_->stream());
   return true;
 }
 
@@ -255,8 +254,7 @@ bool KernelLauncher::copy_out(void* dst, const void* dev_src, size_t bytes) {
 }
 
 void KernelLauncher::release(Operand& op) {
-  if (op.kind == Operand::STAGED && op.dev) (void)hipFree(op.dev);
-  op = Operand{};
+  op = Operand{};  // staging buffers stay with the launcher for the next call
 }
 
 void KernelLauncher::sync() {
@@ -406,7 +404,7 @@ bool KernelLauncher::launch_transform(const std::string& kernel_name, void* in_b
       return report("launch_transform", "element sizes do not match the descriptor"), false;
     Operand in, out;
     bool ok = resolve(in_buffer, count * elem_size, true, in) &&
-              resolve(out_buffer, count * out_elem_size, false, out);
+              resolve(out_buffer, count * out_elem_size, false, out, 1);
     if (ok) ok = run_code(kernel_name, in.dev, out.dev, count, captures, capture_size, true);
     ok = finish(out, ok) && ok;
     release(in);
@@ -430,7 +428,7 @@ bool KernelLauncher::launch_transform(const std::string& kernel_name, void* in_b
   }
   Operand in, out;
   bool ok = resolve(in_buffer, count * elem_size, true, in) &&
-            resolve(out_buffer, count * out_elem_size, false, out);
+            resolve(out_buffer, count * out_elem_size, false, out, 1);
   if (ok)
     ok = run_transform(*ctx_, m, in.dev, out.dev, count, captures, capture_size,
                        arena_host_base(arena_), arena_dev_base(arena_), arena_bytes(arena_)) == PX_OK;
@@ -544,7 +542,7 @@ bool KernelLauncher::launch_exclusive_scan(const std::string& scan_kernel,
   if (count == 0) return true;
   Operand in, out;
   bool ok = resolve(input, count * elem_size, true, in) &&
-            resolve(output, count * elem_size, false, out);
+            resolve(output, count * elem_size, false, out, 1);
   if (ok) {
     prof_begin();
     ok = run_scan(*ctx_, s->dtype, in.dev, out.dev, count, true, init) == PX_OK;
@@ -604,7 +602,7 @@ bool KernelLauncher::launch_compact(const std::string& flags_kernel, const std::
   const bool in_place = input == output;
   Operand in, out;
   bool ok = resolve(input, count * elem_size, true, in) &&
-            (in_place || resolve(output, count * elem_size, false, out));
+            (in_place || resolve(output, count * elem_size, false, out, 1));
   Operand& dst = in_place ? in : out;
   if (ok) {
     prof_begin();
@@ -673,7 +671,7 @@ size_t KernelLauncher::launch_mismatch(const std::string& kernel_name, void* a, 
   const px_desc* d = find(kernel_name, "launch_mismatch");
   if (!d || count == 0 || !size_ok(d->dtype, elem_size)) return count;
   Operand oa, ob;
-  bool ok = resolve(a, count * elem_size, true, oa) && resolve(b, count * elem_size, true, ob);
+  bool ok = resolve(a, count * elem_size, true, oa) && resolve(b, count * elem_size, true, ob, 1);
   uint64_t idx = count;
   if (ok) ok = run_mismatch(*ctx_, d->dtype, oa.dev, ob.dev, count, dev_word_) == PX_OK;
   if (ok) ok = copy_out(&idx, dev_word_, 8);

@@ -442,8 +442,41 @@ void pxs_free(pxs_server_t s, void* d_ptr) {
   s->cv_work.notify_all();
 }
 
+namespace {
+// pxs_submit with the server's lock held (no wake-up of the worker).
+int64_t submit_locked(pxs_server_t s, const pxs_request* req);
+}  // namespace
+
 int64_t pxs_submit(pxs_server_t s, const pxs_request* req) {
   if (!s || !req) return PX_EARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int64_t id = submit_locked(s, req);
+  if (id >= 0) {
+    s->wake = true;
+    s->cv_work.notify_all();
+  }
+  return id;
+}
+
+int pxs_submit_batch(pxs_server_t s, const pxs_request* reqs, size_t n, int64_t* ids) {
+  if (!s || (n && (!reqs || !ids))) return PX_EARG;
+  int bad = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t i = 0; i < n; ++i) {
+      ids[i] = submit_locked(s, &reqs[i]);
+      if (ids[i] < 0) ++bad;
+    }
+    s->wake = true;
+  }
+  s->cv_work.notify_all();
+  return bad;
+}
+
+}  // extern "C"
+
+namespace {
+int64_t submit_locked(pxs_server_t s, const pxs_request* req) {
   if (req->dtype < 0 || req->dtype >= PX_NUM_DTYPES) return PX_EARG;
   if (req->op != PX_OP_REDUCE && req->op != PX_OP_SCAN && req->op != PX_OP_SORT &&
       req->op != PX_OP_FLAGS)
@@ -452,7 +485,6 @@ int64_t pxs_submit(pxs_server_t s, const pxs_request* req) {
                                  req->flags.fn >= PX_NUM_PREDS))
     return PX_EARG;
   const size_t es = pxk::dtype_size(req->dtype);
-  std::lock_guard<std::mutex> lk(s->mu);
   if (req->n && (!s->pool.contains(req->d_in) ||
                  !s->pool.contains(static_cast<char*>(req->d_in) + req->n * es - 1))) {
     parallax::set_last_error(PX_EARG, "pxs_submit: input outside the server's pool");
@@ -497,10 +529,11 @@ int64_t pxs_submit(pxs_server_t s, const pxs_request* req) {
   s->reqs.emplace(id, q);
   s->pending.push_back(id);
   ++s->submitted;
-  s->wake = true;
-  s->cv_work.notify_all();
   return id;
 }
+}  // namespace
+
+extern "C" {
 
 int pxs_wait(pxs_server_t s, int64_t id, double timeout_ms) {
   if (!s) return PX_EARG;

@@ -87,6 +87,8 @@ Tune load() {
     else if (k == "seg_hist_bpc" && v >= 1 && v <= 16) t.seg_hist_bpc = v;
     else if (k == "seg_radix_min" && v >= 1) t.seg_radix_min = static_cast<uint64_t>(v);
     else if (k == "seg_radix_block" && (v == 512 || v == 1024)) t.seg_radix_block = v;
+    else if (k == "copy_bpc" && v >= 1 && v <= 32) t.copy_bpc = v;
+    else if (k == "copy_unroll" && (v == 1 || v == 2 || v == 4 || v == 8)) t.copy_unroll = v;
   }
   for (int d = 0; d < PX_NUM_DTYPES; ++d) {
     TileCfg a{scan.block ? scan.block : t.scan[d].block, scan.k ? scan.k : t.scan[d].k};

@@ -79,6 +79,7 @@ _proto("pxd_ctx_stream", _vp, _vp)
 _proto("pxd_sync", _i, _vp)
 _proto("pxd_timer_begin", _i, _vp)
 _proto("pxd_timer_end", _i, _vp, ctypes.POINTER(ctypes.c_double))
+_proto("pxd_copy", _i, _vp, _vp, _vp, _sz)
 _proto("pxd_reduce", _i, _vp, _i, _vp, _sz, _vp)
 _proto("pxd_transform_reduce", _i, _vp, _dp, _vp, _sz, _vp)
 _proto("pxd_inclusive_scan", _i, _vp, _i, _vp, _vp, _sz)
@@ -130,6 +131,7 @@ _proto("pxs_destroy", None, _vp)
 _proto("pxs_alloc", _vp, _vp, _sz)
 _proto("pxs_free", None, _vp, _vp)
 _proto("pxs_submit", ctypes.c_int64, _vp, ctypes.POINTER(PxsRequest))
+_proto("pxs_submit_batch", _i, _vp, ctypes.POINTER(PxsRequest), _sz, ctypes.POINTER(ctypes.c_int64))
 _proto("pxs_wait", _i, _vp, ctypes.c_int64, ctypes.c_double)
 _proto("pxs_result", _i, _vp, ctypes.c_int64, ctypes.POINTER(_vp), _u64p, ctypes.POINTER(ctypes.c_double))
 _proto("pxs_release", _i, _vp, ctypes.c_int64)

@@ -241,3 +241,49 @@ def test_abi_results_to_device_pointers_back_to_back():
     got_i = raw.reshape(32, 16)[:, 8:].copy().view(np.int64).ravel()
     assert np.all(got_f == np.float32(f.astype(np.int64).sum()))
     assert np.all(got_i == pxo.reduce(i))
+
+
+def test_variadic_launch_transform(gpu_ctx):
+    """parallax_kernel_launch_transform(k, in, out, count, elem_size) (reference
+    runtime.h:35, cache.cpp:147-190: the variadic (void*, void*, size_t, size_t) form)."""
+    a = np.arange(-5000, 5000, dtype=np.int32)
+    kt = load(make_desc(C.OP_TRANSFORM, PX_I32, aux=PX_I32, fn=C.MAP_AXPB, c0=3, c1=-7))
+    out = np.zeros_like(a)
+    lib.parallax_kernel_launch_transform(ctypes.c_void_p(kt), ctypes.c_void_p(a.ctypes.data),
+                                         ctypes.c_void_p(out.ctypes.data), ctypes.c_size_t(a.size),
+                                         ctypes.c_size_t(4))
+    d = make_desc(C.OP_TRANSFORM, PX_I32, aux=PX_I32, fn=C.MAP_AXPB, c0=3, c1=-7)
+    assert np.array_equal(out, pxo.transform(d, a, np.int32))
+    # device memory in and out
+    di, do = DeviceBuffer.from_array(a), DeviceBuffer(a.nbytes)
+    lib.parallax_kernel_launch_transform(ctypes.c_void_p(kt), ctypes.c_void_p(di.ptr), ctypes.c_void_p(do.ptr),
+                                         ctypes.c_size_t(a.size), ctypes.c_size_t(4))
+    assert np.array_equal(do.download(np.int32, a.size), out)
+
+
+def test_register_buffer_data_path(gpu_ctx):
+    """parallax_register_buffer (reference runtime.h:145, C++ linkage; cache.cpp:394-402):
+    calls on a registered host range see its current contents every time and write their
+    results back to it (the reference copied a registered buffer only once and never
+    copied results back, unified_buffer.cpp:266-310)."""
+    reg = getattr(lib, "_Z24parallax_register_bufferPvm")
+    reg.restype = ctypes.c_bool
+    reg.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    a = np.random.default_rng(4).integers(0, 1 << 30, 100_003).astype(np.uint32)
+    assert reg(a.ctypes.data, a.nbytes)
+    ksum = load(make_desc(C.OP_REDUCE, PX_U32))
+    r = np.zeros(1, dtype=np.uint32)
+    lib.parallax_reduce(ksum, a.ctypes.data, a.size, 4, r.ctypes.data)
+    assert r[0] == pxo.reduce(a)
+    a[::7] += 3  # rewritten after registration: the next call sees the new values
+    lib.parallax_reduce(ksum, a.ctypes.data, a.size, 4, r.ctypes.data)
+    assert r[0] == pxo.reduce(a)
+    ref = pxo.sort(a)
+    ksort = load(make_desc(C.OP_SORT, PX_U32))
+    lib.parallax_sort(ksort, a.ctypes.data + 0, a.size, 4)  # in place: the result reaches a
+    assert np.array_equal(a, ref)
+    # a pointer inside the registered range
+    b = a[1000:2000].copy()
+    a[1000:2000] = b[::-1]
+    lib.parallax_sort(ksort, a.ctypes.data + 4000, 1000, 4)
+    assert np.array_equal(a[1000:2000], np.sort(b))

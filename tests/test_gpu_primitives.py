@@ -575,3 +575,24 @@ def test_beyond_2p32_elements(gpu_ctx):
         lib.pxd_free(src)
         lib.pxd_free(dst)
         lib.pxd_free(word)
+
+
+@pytest.mark.parametrize("nbytes", [0, 1, 15, 16, 17, 4096, 1_000_003, 1 << 26])
+@pytest.mark.parametrize("offs", [(0, 0), (3, 3), (1, 2), (4, 8), (8, 0)])
+def test_copy_kernel(gpu_ctx, nbytes, offs):
+    """pxd_copy / pxd_memcpy_d2d (the streaming-copy kernel): exact bytes for every
+    alignment pair, nothing written outside the destination range."""
+    rng = np.random.default_rng(nbytes + 7 * offs[0] + offs[1])
+    src = rng.integers(0, 256, nbytes + 64, dtype=np.uint8)
+    sb = DeviceBuffer.from_array(src)
+    db = DeviceBuffer.from_array(np.full(nbytes + 64, 0xAB, dtype=np.uint8))
+    check(lib.pxd_copy(gpu_ctx.h, db.addr(offs[1]), sb.addr(offs[0]), nbytes), "copy")
+    gpu_ctx.sync()
+    got = db.download(np.uint8, nbytes + 64)
+    assert got[offs[1]:offs[1] + nbytes].tobytes() == src[offs[0]:offs[0] + nbytes].tobytes()
+    assert np.all(got[:offs[1]] == 0xAB) and np.all(got[offs[1] + nbytes:] == 0xAB)
+    db.upload(np.full(nbytes + 64, 0xCD, dtype=np.uint8))
+    check(lib.pxd_memcpy_d2d(db.addr(offs[1]), sb.addr(offs[0]), nbytes), "memcpy d2d")
+    got = db.download(np.uint8, nbytes + 64)
+    assert got[offs[1]:offs[1] + nbytes].tobytes() == src[offs[0]:offs[0] + nbytes].tobytes()
+    assert np.all(got[:offs[1]] == 0xCD) and np.all(got[offs[1] + nbytes:] == 0xCD)
