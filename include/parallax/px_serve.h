@@ -34,7 +34,7 @@ extern "C" {
 typedef struct pxs_server* pxs_server_t;
 
 typedef struct pxs_config {
-  int slots;            /* launches in flight at once (streams); default 4 */
+  int slots;            /* launches in flight at once (streams); default 8 */
   size_t hbm_budget;    /* bytes of the device pool (inputs + outputs); default 64 GiB */
   int max_batch;        /* most requests fused into one launch; default 4096 */
   double max_wait_us;   /* how long a lone request may wait for batch-mates; default 0 */

@@ -173,18 +173,33 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   if (gtid < head) count(data[gtid]);
   const V* vp = reinterpret_cast<const V*>(data + head);
   const uint64_t nv = (n - head) / VEC;
-  constexpr int UL = 4;  // vector loads in flight per thread; one digit test per group
+  // UL vector loads per group, one digit test per group; the next group's loads are
+  // issued before this group's LDS adds, so the adds overlap the memory latency
+  constexpr int UL = 4;
   uint64_t v = gtid;
-  for (; v + (UL - 1) * gstride < nv; v += UL * gstride) {
-    V xs[UL];
+  V cur[UL], nxt[UL];
+  bool have = v + (UL - 1) * gstride < nv;
+  if (have) {
 #pragma unroll
-    for (int u = 0; u < UL; ++u) xs[u] = ld_stream(vp + v + u * gstride);
+    for (int u = 0; u < UL; ++u) cur[u] = ld_stream(vp + v + u * gstride);
+  }
+  while (have) {
+    const uint64_t vn = v + UL * gstride;
+    const bool have_n = vn + (UL - 1) * gstride < nv;
+    if (have_n) {
+#pragma unroll
+      for (int u = 0; u < UL; ++u) nxt[u] = ld_stream(vp + vn + u * gstride);
+    }
     U k[UL * VEC];
 #pragma unroll
     for (int u = 0; u < UL; ++u)
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) k[u * VEC + j] = RadixKey<T>::enc(xs[u].v[j]);
+      for (int j = 0; j < VEC; ++j) k[u * VEC + j] = RadixKey<T>::enc(cur[u].v[j]);
     hist_add<D>(hw, k, lane);
+#pragma unroll
+    for (int u = 0; u < UL; ++u) cur[u] = nxt[u];
+    v = vn;
+    have = have_n;
   }
   for (; v < nv; v += gstride) {
     const V x = ld_stream(vp + v);
@@ -609,16 +624,21 @@ hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t 
   if (n >= (1ull << 32) || ntiles * kBins > s.status_words || s.partial_bytes < n * sizeof(T) || !tmp)
     return hipErrorInvalidValue;
   const uint32_t cus = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
-  // Histograms: 256-thread blocks, two per CU, no more blocks than 16 vectors per thread
-  // need.  The LDS atomics bound this kernel (about 21 cycles per wave-wide add with
-  // random digits): one block per CU or 1024-thread blocks run as fast; 3, 4 or 6 per
-  // CU are slower (each block adds D * 256 counters to the same global words).
-  constexpr int HB = 256;
-  const uint64_t hist_per_cu = tune().radix_hist_bpc > 0 ? static_cast<uint64_t>(tune().radix_hist_bpc) : 2;
+  // Histograms: no more blocks than 16 vectors per thread need.  One 512-thread block per CU (measured, 2^26 u32 / 2^25 f64: 121 / 128 us; 256
+  // threads x 1 or 2 per CU 132 / 137 and 154 / 161 us, 1024 x 1 145 / 151, 512 x 2
+  // 193 / 207): every block adds its D * 256 counts to the control block with
+  // device-scope atomics at the end, so more blocks cost more than they hide.
+  const int HB = (tune().radix_hist_block == 256 || tune().radix_hist_block == 1024) ? tune().radix_hist_block : 512;
+  const uint64_t hist_per_cu = tune().radix_hist_bpc > 0 ? static_cast<uint64_t>(tune().radix_hist_bpc) : 1;
   const uint64_t hist_work = (n + HB * 16ull * (16 / sizeof(T)) - 1) / (HB * 16ull * (16 / sizeof(T)));
   const uint32_t hist_grid = static_cast<uint32_t>(std::min<uint64_t>(hist_per_cu * cus, hist_work ? hist_work : 1));
   hook("sort_hist", 0);
-  k_radix_hist<T, HB><<<hist_grid, HB, 0, stream>>>(data, n, s.ctl);
+  if (HB == 1024)
+    k_radix_hist<T, 1024><<<hist_grid, 1024, 0, stream>>>(data, n, s.ctl);
+  else if (HB == 512)
+    k_radix_hist<T, 512><<<hist_grid, 512, 0, stream>>>(data, n, s.ctl);
+  else
+    k_radix_hist<T, 256><<<hist_grid, 256, 0, stream>>>(data, n, s.ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hook("sort_hist", 1);

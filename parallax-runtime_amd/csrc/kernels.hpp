@@ -62,7 +62,8 @@ struct Tune {
   // of 4 bytes, 2^20 of 8 bytes, the measured crossovers with the bitonic path)
   uint64_t sort_radix_min = ~0ull;
   int radix_block = 0;        // threads per radix digit-pass block (256, 512, 1024; 0: by size)
-  int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 2)
+  int radix_hist_bpc = 0;     // radix histogram blocks per CU (0: default, 1)
+  int radix_hist_block = 0;   // radix histogram block: 256, 512 or 1024 threads (0: default, 512)
   int seg_hist_bpc = 0;       // segmented radix histogram blocks per CU (0: default, 4)
   uint64_t seg_radix_min = 0;  // segments longer than this use the segmented radix sort (0: default, 1,024 / 512 keys)
   int seg_radix_block = 0;     // segmented radix digit-pass block, 512 or 1024 (0: chosen per call)

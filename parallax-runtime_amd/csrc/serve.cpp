@@ -364,7 +364,7 @@ extern "C" {
 
 void pxs_config_default(pxs_config* cfg) {
   if (!cfg) return;
-  cfg->slots = 4;
+  cfg->slots = 8;
   cfg->hbm_budget = 64ull << 30;
   cfg->max_batch = 4096;
   cfg->max_wait_us = 0.0;

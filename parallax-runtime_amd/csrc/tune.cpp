@@ -84,6 +84,7 @@ Tune load() {
     else if (k == "sort_radix_min" && v >= 0) t.sort_radix_min = static_cast<uint64_t>(v);
     else if (k == "radix_block" && (v == 256 || v == 512 || v == 1024)) t.radix_block = v;
     else if (k == "radix_hist_bpc" && v >= 1 && v <= 16) t.radix_hist_bpc = v;
+    else if (k == "radix_hist_block" && (v == 256 || v == 512 || v == 1024)) t.radix_hist_block = v;
     else if (k == "seg_hist_bpc" && v >= 1 && v <= 16) t.seg_hist_bpc = v;
     else if (k == "seg_radix_min" && v >= 1) t.seg_radix_min = static_cast<uint64_t>(v);
     else if (k == "seg_radix_block" && (v == 512 || v == 1024)) t.seg_radix_block = v;
