@@ -41,6 +41,7 @@ __device__ __forceinline__ void write_run(T* dst, const T* src, C count) {
   if (threadIdx.x < head) dst[threadIdx.x] = src[threadIdx.x];
   const uint32_t nvec = static_cast<uint32_t>((count - head) / VEC);
   V* vd = reinterpret_cast<V*>(dst + head);
+#pragma unroll 1
   for (uint32_t v = threadIdx.x; v < nvec; v += BLOCK) {
     V y;
 #pragma unroll
@@ -72,6 +73,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
 
   __shared__ S s_stage[TILE];
   __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_next;
   __shared__ uint32_t s_epoch;
   __shared__ C s_w[NW];
   __shared__ C s_prefix;
@@ -84,7 +86,11 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
   // thread element e = k*VEC + j sits at rel = base_rel(k) + j; the segment covers
   // [lo_r, hi_r) of the window (lo_r = -1 when the segment starts before it; an empty
   // range when it ends before the window or starts after it).
-  auto base_rel = [&](int k) { return ((wid * K + k) * kWave + lane) * VEC; };
+  // `tid_o` is threadIdx.x made opaque once per tile: position arithmetic is redone per
+  // tile rather than hoisted out of the tile loop, where its 32 invariants would be
+  // live (and spilled) across the whole loop.
+  int tid_o = static_cast<int>(threadIdx.x);
+  auto base_rel = [&](int k) { return ((tid_o / kWave * K + k) * kWave + tid_o % kWave) * VEC; };
 
   struct Tile {
     T x[E];
@@ -97,8 +103,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
   // the split of a partition (read once: the counting pass ran before on this stream)
   const uint64_t split = (MODE == kModeMisplaced) ? *num_true_dev : 0;
 
-  // Loads, flags and ranks the tile (ends with the block's counts in s_w read out).
-  auto load_rank = [&](Tile& u, const TileInfo& ti) {
+  // Raw loads of the tile and its valid bits (no LDS, no barrier: may run ahead).
+  auto load = [&](Tile& u, const TileInfo& ti) {
     u.w0 = ti.w0;
     u.lo = ti.lo;
     const int64_t dl = ti.lo - u.w0, dh = ti.hi - u.w0;
@@ -135,7 +141,11 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
           if (r0 + j >= u.lo_r && r0 + j < u.hi_r) u.valid |= 1u << (k * VEC + j);
       }
     }
+  };
 
+  // Flags and ranks the loaded tile (ends with the block's counts in s_w read out);
+  // thread 0 takes the block's next ticket into s_next on the way.
+  auto rank = [&](Tile& u) {
     // ---- flags: the predicate (or adjacent difference) on valid positions
     if constexpr (UNIQUE) {
       u.flags = 0;
@@ -181,6 +191,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
       run += read_lane(incl, kWave - 1);
     }
     if (lane == 0) s_w[wid] = run;
+    if (threadIdx.x == 0)
+      s_next = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     u.wave_off = 0;
     for (int j = 0; j < wid; ++j) u.wave_off += s_w[j];
@@ -227,52 +239,77 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
     }
   };
 
-  if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
+  if (threadIdx.x == 0) {
+    s_epoch = ld_agent32(&ctl->epoch);
+    s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const uint32_t epoch = s_epoch;
 
-  // Persistent loop over tickets; a tile waits only on lower tickets, all of which are
-  // held by running blocks, so every wait ends.
-  while (true) {
-    if (threadIdx.x == 0)
-      s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t epoch = s_epoch;
-    if (tile >= g.ntiles) break;
-    const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
-
+  // Persistent loop over tickets.  The block takes its next ticket while ranking the
+  // current tile and issues that tile's loads once the current one is staged in LDS,
+  // before the look-back and the writes, so their latency overlaps.  A block holds at most two tickets and finishes the lower
+  // first; the lowest unfinished ticket waits only on finished ones, so every wait
+  // ends.  In place (out == in) stays safe: a tile's outputs land below its own end,
+  // and its look-back ends only after every earlier tile has loaded.
+  uint32_t tile = s_tile;
+  if (tile < g.ntiles) {
+    TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
     Tile a;
-    load_rank(a, ti);
-    const C tile_kept = a.kept;
-    // publish the tile's count before staging, so later tiles' look-backs need not wait
-    if (threadIdx.x == 0 && tile != ti.first_tile) publish<C>(status, tile, epoch, kStAgg, tile_kept);
-    stage(a);
-
-    // ---- kept before this tile (look-back), then the tile's count
-    if (wid == 0) {
-      C prefix;
-      if (tile == ti.first_tile) {
-        prefix = 0;
-      } else {
-        prefix = lookback<C>(status, tile, ti.first_tile, epoch);
+    load(a, ti);
+    while (true) {
+      asm volatile("" : "+v"(tid_o));
+      rank(a);
+      const C tile_kept = a.kept;
+      // publish the tile's count before staging, so later tiles' look-backs need not wait
+      if (threadIdx.x == 0 && tile != ti.first_tile) publish<C>(status, tile, epoch, kStAgg, tile_kept);
+      stage(a);
+      // the tile's values now sit in LDS: their registers take the next tile's loads
+      __builtin_amdgcn_sched_barrier(0);
+      const uint32_t next = s_next;
+      const bool more = next < g.ntiles;
+      TileInfo tn = ti;
+      Tile b;
+      // wave 0 looks back: it issues its own loads of the next tile once the look-back
+      // is done (a poll's wait for its load would also wait for every load before it)
+      if (more) {
+        tn = tile_info<VEC, TILE>(g, next, in_shift);
+        asm volatile("" : "+v"(tid_o));
+        if (wid != 0) load(b, tn);
       }
-      if (tile != ti.last_tile && lane == 0)
-        publish<C>(status, tile, epoch, kStIncl, prefix + tile_kept);
-      if (lane == 0) {
-        s_prefix = prefix;
-        if (tile == ti.last_tile) {
-          const uint64_t total = static_cast<uint64_t>(prefix) + tile_kept;
-          if (counts) counts[ti.seg] = total;
-          else ctl->result = total;
+
+      // ---- kept before this tile (look-back), then the tile's count
+      if (wid == 0) {
+        C prefix;
+        if (tile == ti.first_tile) {
+          prefix = 0;
+        } else {
+          prefix = lookback<C>(status, tile, ti.first_tile, epoch);
         }
+        if (tile != ti.last_tile && lane == 0)
+          publish<C>(status, tile, epoch, kStIncl, prefix + tile_kept);
+        if (lane == 0) {
+          s_prefix = prefix;
+          if (tile == ti.last_tile) {
+            const uint64_t total = static_cast<uint64_t>(prefix) + tile_kept;
+            if (counts) counts[ti.seg] = total;
+            else ctl->result = total;
+          }
+        }
+        if (more) load(b, tn);
       }
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // ---- coalesced writes of the staged run(s)
-    const uint64_t P = s_prefix;
-    const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
-    write(a, ti, seg_out, P);
-    __syncthreads();
+      // ---- coalesced writes of the staged run(s)
+      const uint64_t P = s_prefix;
+      const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
+      write(a, ti, seg_out, P);
+      __syncthreads();
+      if (!more) break;
+      a = b;
+      tile = next;
+      ti = tn;
+    }
   }
   if (threadIdx.x == 0) block_end(ctl, s_epoch);
 }

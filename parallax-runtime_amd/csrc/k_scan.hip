@@ -139,6 +139,83 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     }
   };
 
+  // Two tiles in registers (this one and the next) fit 128 VGPRs when the chunk carries
+  // are 32-bit or the tile has 4 chunks; wider carries over 8 chunks keep one tile.
+  constexpr bool kPrefetch = S == 1 && 8 * KR + KR * static_cast<int>(sizeof(A)) / 4 <= 72;
+  if constexpr (kPrefetch) {
+    // One sub-tile per ticket: the block takes its next ticket and issues that tile's
+    // loads before the look-back of the current one, so the look-back's round trips
+    // overlap the next tile's memory latency.  A block holds at most two tickets and
+    // finishes the lower first; the lowest unfinished ticket waits only on finished
+    // ones, so every wait still ends.  Wave totals are double-buffered by tile parity:
+    // a wave may publish the next tile's total while another still reads this one's.
+    __shared__ uint32_t s_next;
+    __shared__ A s_w2[2][NW];
+    if (threadIdx.x == 0) {
+      s_epoch = ld_agent32(&ctl->epoch);
+      s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const uint32_t epoch = s_epoch;
+    uint32_t tile = s_tile;
+    if (tile < g.ntiles) {
+      TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
+      T x[KR][VEC];
+      load(x, ti, 0);
+      for (uint32_t it = 0;; ++it) {
+        const int par = static_cast<int>(it & 1u);
+        // the wave total now; the chunk carries (the same scans) again after the
+        // look-back, so they hold no registers while the next tile's loads are in flight
+        A tex[KR];
+        const A run = local_scan(x, tex);
+        if (lane == 0) s_w2[par][wid] = run;
+        if (threadIdx.x == 0)
+          s_next = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t next = s_next;
+        const bool more = next < g.ntiles;
+        TileInfo tn = ti;
+        T xn[KR][VEC];
+        // wave 0 looks back: it issues its own loads of the next tile once the look-back
+        // is done (a poll's wait for its load would also wait for every load before it)
+        if (more) {
+          tn = tile_info<VEC, TILE>(g, next, in_shift);
+          if (wid != 0) load(xn, tn, 0);
+        }
+        if (wid == 0) {
+          A tile_tot = s_w2[par][0];
+#pragma unroll
+          for (int j = 1; j < NW; ++j) tile_tot = acc_add(tile_tot, s_w2[par][j]);
+          A prefix;
+          if (tile == ti.first_tile) {
+            prefix = EXCL ? init : acc_zero<A>();
+          } else {
+            if (lane == 0) publish<A>(status, tile, epoch, kStAgg, tile_tot);
+            prefix = lookback<A>(status, tile, ti.first_tile, epoch);
+          }
+          if (tile != ti.last_tile && lane == 0)
+            publish<A>(status, tile, epoch, kStIncl, acc_add(prefix, tile_tot));
+          if (lane == 0) s_prefix = prefix;
+          if (more) load(xn, tn, 0);
+        }
+        __syncthreads();
+        A w = s_prefix;
+        for (int j = 0; j < wid; ++j) w = acc_add(w, s_w2[par][j]);
+        local_scan(x, tex);
+        emit(x, tex, w, ti, 0);
+        if (!more) break;
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) x[k][j] = xn[k][j];
+        tile = next;
+        ti = tn;
+      }
+    }
+    if (threadIdx.x == 0) block_end(ctl, epoch);
+    return;
+  }
+
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
 
   // Persistent loop over tickets; a tile waits only on lower tickets, all of which are
