@@ -813,7 +813,7 @@ def served(args):
             "dtype": "mixed(i32,u64,f32,f64)",
             "config": {"workload": f"served mixed stream: {nreq} requests (30% reduce, 30% scan, 25% sort, "
                                    "15% copy_if), sizes log-uniform [2^10,2^24], dtypes {i32,u64,f32,f64}, "
-                                   f"seed 4, 64 GB pool, {cfg.slots} slots", "scale": args.scale},
+                                   f"seed 4, 64 GB pool, {cfg.slots} slots on {cfg.lanes} stream(s)", "scale": args.scale},
             "req_per_s": {"served_saturated": round(sat.req_per_s, 1), "served_poisson": round(poi.req_per_s, 1),
                           "saturated_passes": len(passes), "saturated_min": round(passes[0].req_per_s, 1),
                           "saturated_max": round(passes[-1].req_per_s, 1),

@@ -4,9 +4,12 @@
  * NO COUNTERPART IN THE REFERENCE (its primitives are synchronous calls only).  For
  * callers that issue many primitive calls of mixed op, element type and size as they
  * arrive: buffers stay resident in a device pool the server owns, requests are admitted
- * against an HBM budget, requests that are waiting when a slot (a stream with its own
- * scratch) frees up are fused into ONE segmented launch per (op, dtype, predicate)
- * group, and several groups run concurrently on different slots (continuous batching).
+ * against an HBM budget, requests that are waiting when a slot (one batch in flight, with
+ * its own scratch) frees up are fused into ONE segmented launch per (op, dtype,
+ * predicate) group, and batches are queued on the device while earlier ones run and new
+ * requests keep arriving (continuous batching).  Slots share `lanes` streams (default
+ * one): batches of one lane run back to back in launch order, so the device never waits
+ * for the host between them and two whole-device look-back kernels never share the CUs.
  * A request's result does not depend on what it was batched with: each request is one
  * segment with the same tiling it would get alone.
  *
@@ -34,10 +37,11 @@ extern "C" {
 typedef struct pxs_server* pxs_server_t;
 
 typedef struct pxs_config {
-  int slots;            /* launches in flight at once (streams); default 8 */
+  int slots;            /* batches in flight at once; default 8 */
   size_t hbm_budget;    /* bytes of the device pool (inputs + outputs); default 64 GiB */
   int max_batch;        /* most requests fused into one launch; default 4096 */
   double max_wait_us;   /* how long a lone request may wait for batch-mates; default 0 */
+  int lanes;            /* streams the slots share (1..slots); default 1 */
 } pxs_config;
 
 typedef struct pxs_request {

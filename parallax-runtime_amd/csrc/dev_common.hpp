@@ -2,7 +2,9 @@
 //
 //  * element traits: per element type T, the carry type A the kernels accumulate in
 //      int32/uint32 -> uint32 (wrapping), int64/uint64 -> uint64 (wrapping),
-//      float -> double, double -> double-double (hi, lo).
+//      float -> double, double -> double-double (hi, lo),
+//    and the local type L a scan uses inside one wave's chunks (A, except double for
+//    double: only the carries between waves and tiles need the double-double).
 //    Integer sums are exact modulo 2^w in any order; the float carries keep every
 //    reduce / scan result within a few units of the last place of the exact sum.
 //  * wave64 cross-lane steps on DPP (row_shr, row_bcast15/31, quad_perm, row mirrors,
@@ -37,37 +39,53 @@ __device__ __forceinline__ dd dd_add(dd a, dd b) {
   return dd{h, l};
 }
 
+// a + b for a double b: TwoSum of a.hi and b, a.lo folded in, renormalised.
+__device__ __forceinline__ dd dd_add_d(dd a, double b) {
+  const double s = a.hi + b;
+  const double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b - bb);
+  e += a.lo;
+  const double h = s + e;
+  return dd{h, e - (h - s)};
+}
+
 // ---------------------------------------------------------------- carry traits
 template <class T>
 struct Traits;
 template <>
 struct Traits<int32_t> {
   using A = uint32_t;
+  using L = uint32_t;
   static constexpr bool is_float = false;
 };
 template <>
 struct Traits<uint32_t> {
   using A = uint32_t;
+  using L = uint32_t;
   static constexpr bool is_float = false;
 };
 template <>
 struct Traits<int64_t> {
   using A = uint64_t;
+  using L = uint64_t;
   static constexpr bool is_float = false;
 };
 template <>
 struct Traits<uint64_t> {
   using A = uint64_t;
+  using L = uint64_t;
   static constexpr bool is_float = false;
 };
 template <>
 struct Traits<float> {
   using A = double;
+  using L = double;
   static constexpr bool is_float = true;
 };
 template <>
 struct Traits<double> {
   using A = dd;
+  using L = double;
   static constexpr bool is_float = true;
 };
 
@@ -91,6 +109,32 @@ __device__ __forceinline__ uint64_t to_acc(int64_t x) { return static_cast<uint6
 __device__ __forceinline__ uint64_t to_acc(uint64_t x) { return x; }
 __device__ __forceinline__ double to_acc(float x) { return static_cast<double>(x); }
 __device__ __forceinline__ dd to_acc(double x) { return dd{x, 0.0}; }
+
+// Local (in-wave) scan values: the carry type, except double for double.
+template <class T>
+__device__ __forceinline__ auto to_loc(T x) {
+  return to_acc(x);
+}
+template <>
+__device__ __forceinline__ auto to_loc<double>(double x) {
+  return x;
+}
+// A local value as a carry, and a carry plus a local value.
+template <class A, class L>
+__device__ __forceinline__ A loc_to_acc(L v) {
+  if constexpr (sizeof(A) == sizeof(L)) return v;
+  else return dd{v, 0.0};
+}
+__device__ __forceinline__ uint32_t acc_add_loc(uint32_t a, uint32_t b) { return a + b; }
+__device__ __forceinline__ uint64_t acc_add_loc(uint64_t a, uint64_t b) { return a + b; }
+__device__ __forceinline__ double acc_add_loc(double a, double b) { return a + b; }
+__device__ __forceinline__ dd acc_add_loc(dd a, double b) { return dd_add_d(a, b); }
+// The output element of a carry plus a local value (double-double: hi + (lo + r)).
+template <class T, class A, class L>
+__device__ __forceinline__ T out_of(A base, L r) {
+  if constexpr (sizeof(A) == sizeof(L)) return from_acc<T>(acc_add(base, r));
+  else return base.hi + (base.lo + r);
+}
 
 template <class T>
 __device__ __forceinline__ T from_acc(typename Traits<T>::A a);
@@ -173,6 +217,16 @@ __device__ __forceinline__ A wave_incl_scan(A v) {
   v = acc_add(v, dpp_mov<kRowShr8, 0xf, 0xf>(v));
   v = acc_add(v, dpp_mov<kRowBcast15, 0xa, 0xf>(v));
   v = acc_add(v, dpp_mov<kRowBcast31, 0xc, 0xf>(v));
+  return v;
+}
+
+// Inclusive scan inside each 16-lane row (row_shr 1, 2, 4, 8).
+template <class A>
+__device__ __forceinline__ A row_incl_scan(A v) {
+  v = acc_add(v, dpp_mov<kRowShr1, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowShr2, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowShr4, 0xf, 0xf>(v));
+  v = acc_add(v, dpp_mov<kRowShr8, 0xf, 0xf>(v));
   return v;
 }
 

@@ -102,7 +102,9 @@ bool StreamContext::grow(void** p, size_t* cap, size_t need, bool zero) {
       return false;
     }
   }
-  if (zero && hipMemset(*p, 0, ncap) != hipSuccess) return false;
+  // cleared in stream order, before any later launch on this stream reads it (never
+  // inside a graph capture: a frozen context does not grow)
+  if (zero && hipMemsetAsync(*p, 0, ncap, stream_) != hipSuccess) return false;
   *cap = ncap;
   return true;
 }
@@ -450,7 +452,7 @@ std::vector<pxk::TileCfg> seg_tile_candidates(int op, int dtype) {
   } else {
     for (pxk::TileCfg t : {pxk::TileCfg{256, 4}, pxk::TileCfg{256, 8}, pxk::TileCfg{512, 8}, pxk::TileCfg{1024, 4},
                            pxk::TileCfg{1024, 8}})
-      if (pxk::tile_cfg_supported(t, op) && !(dtype == PX_F64 && t.block == 1024 && t.k == 8)) v.push_back(t);
+      if (pxk::tile_cfg_supported(t, op)) v.push_back(t);
   }
   return v;
 }

@@ -14,6 +14,16 @@
 // nearest inclusive prefix), publishes its inclusive prefix, and the tile writes its
 // outputs with non-temporal 16-byte stores.  Tiles of a batched (segmented) call look
 // back only inside their own segment.
+//
+// Arithmetic: everything inside a wave's chunks runs in the local type L; the wave
+// totals are combined into per-wave carries by one 16-lane scan in the carry type A, and
+// the look-back runs in A.  For double (L = double, A = double-double) an output is
+// hi + (lo + local), where local = the element's prefix inside its wave; it passes
+// through at most 1 (vector pair) + 6 (wave scan) + KR - 1 (chunk carry) + 1 (vector
+// element) + 2 (output) roundings, each bounded by eps times the sum of |x| over the
+// prefix, and the carries add about eps^2 -- within 2 ceil(log2 n) eps sum|x| (n = the
+// segment's length; a short segment fills fewer lanes and chunks, and sums of zeros are
+// exact).
 #include <cstring>
 
 #include "dev_common.hpp"
@@ -30,6 +40,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
                                               typename Traits<T>::A init, uint64_t* status,
                                               Ctl* ctl) {
   using A = typename Traits<T>::A;
+  using L = typename Traits<T>::L;
   constexpr int VEC = 16 / sizeof(T);
   // A ticket covers S sub-tiles of KR register chunks each; all but the last sub-tile
   // wait in LDS (lane-private slots) while the last is scanned in registers, so one
@@ -40,13 +51,14 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
   constexpr int SUB = BLOCK * KR * VEC;
   constexpr int TILE = BLOCK * K * VEC;
   constexpr int NW = BLOCK / kWave;
+  static_assert(NW <= 16, "the wave totals are scanned in one 16-lane DPP row");
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
   V* vout = reinterpret_cast<V*>(reinterpret_cast<uintptr_t>(out) & ~uintptr_t(15));
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
-  __shared__ A s_w[S][NW];
+  __shared__ L s_w[S][NW];
   __shared__ A s_prefix;
   __shared__ V s_stash[S > 1 ? (S - 1) * KR * BLOCK : 1];
 
@@ -84,16 +96,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     }
   };
 
-  // Per-chunk wave scans carried across the wave's chunks; returns the wave total.
-  auto local_scan = [&](const T (&x)[KR][VEC], A (&tex)[KR]) {
-    A run = acc_zero<A>();
+  // Per-chunk wave scans carried across the wave's chunks (local type); returns the
+  // wave total, tex[k] = this lane's exclusive prefix inside the wave.
+  auto local_scan = [&](const T (&x)[KR][VEC], L (&tex)[KR]) {
+    L run = acc_zero<L>();
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      A c = to_acc(x[k][0]);
+      L c = to_loc(x[k][0]);
 #pragma unroll
-      for (int j = 1; j < VEC; ++j) c = acc_add(c, to_acc(x[k][j]));
-      const A incl = wave_incl_scan(c);
-      const A tot = read_lane(incl, kWave - 1);
+      for (int j = 1; j < VEC; ++j) c = acc_add(c, to_loc(x[k][j]));
+      const L incl = wave_incl_scan(c);
+      const L tot = read_lane(incl, kWave - 1);
       tex[k] = acc_add(run, wave_shift_right1(incl));
       run = acc_add(run, tot);
     }
@@ -102,30 +115,44 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
 
   // Outputs of one sub-tile (16-byte stores when the output lines up with the input
   // vectors); `base` is the carry into this wave's first chunk.
-  auto emit = [&](const T (&x)[KR][VEC], const A (&tex)[KR], A base, const TileInfo& ti, int sub) {
+  auto emit = [&](const T (&x)[KR][VEC], const L (&tex)[KR], A base, const TileInfo& ti, int sub) {
     const int64_t odelta = ti.out_delta - in_shift;  // output element = position + odelta
     const bool vec_ok = ((out_shift + odelta) & (VEC - 1)) == 0;
     const int64_t b0 = ti.w0 + static_cast<int64_t>(sub) * SUB;
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-      A r = acc_add(base, tex[k]);
       T xv[VEC];
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
         xv[j] = x[k][j];
         // Widening carries: convert again here rather than keep the widened values of
         // the whole tile alive from the local scan (twice the registers).
-        if constexpr (sizeof(A) > sizeof(T)) asm volatile("" : "+v"(xv[j]));
+        if constexpr (sizeof(L) > sizeof(T)) asm volatile("" : "+v"(xv[j]));
       }
       V y;
+      if constexpr (sizeof(A) == sizeof(L)) {
+        A r = acc_add(base, tex[k]);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        if (EXCL) {
-          y.v[j] = from_acc<T>(r);
-          r = acc_add(r, to_acc(xv[j]));
-        } else {
-          r = acc_add(r, to_acc(xv[j]));
-          y.v[j] = from_acc<T>(r);
+        for (int j = 0; j < VEC; ++j) {
+          if (EXCL) {
+            y.v[j] = from_acc<T>(r);
+            r = acc_add(r, to_acc(xv[j]));
+          } else {
+            r = acc_add(r, to_acc(xv[j]));
+            y.v[j] = from_acc<T>(r);
+          }
+        }
+      } else {
+        L r = tex[k];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          if (EXCL) {
+            y.v[j] = out_of<T>(base, r);
+            r = acc_add(r, to_loc(xv[j]));
+          } else {
+            r = acc_add(r, to_loc(xv[j]));
+            y.v[j] = out_of<T>(base, r);
+          }
         }
       }
       const int64_t p0 = b0 + ((static_cast<int64_t>(wid) * KR + k) * kWave + lane) * VEC;
@@ -141,7 +168,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
 
   // Two tiles in registers (this one and the next) fit 128 VGPRs when the chunk carries
   // are 32-bit or the tile has 4 chunks; wider carries over 8 chunks keep one tile.
-  constexpr bool kPrefetch = S == 1 && 8 * KR + KR * static_cast<int>(sizeof(A)) / 4 <= 72;
+  constexpr bool kPrefetch = S == 1 && 8 * KR + KR * static_cast<int>(sizeof(L)) / 4 <= 72;
   if constexpr (kPrefetch) {
     // One sub-tile per ticket: the block takes its next ticket and issues that tile's
     // loads before the look-back of the current one, so the look-back's round trips
@@ -150,7 +177,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     // ones, so every wait still ends.  Wave totals are double-buffered by tile parity:
     // a wave may publish the next tile's total while another still reads this one's.
     __shared__ uint32_t s_next;
-    __shared__ A s_w2[2][NW];
+    __shared__ L s_w2[2][NW];
+    __shared__ A s_off[NW];  // carry into each wave's first chunk
     if (threadIdx.x == 0) {
       s_epoch = ld_agent32(&ctl->epoch);
       s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -166,8 +194,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
         const int par = static_cast<int>(it & 1u);
         // the wave total now; the chunk carries (the same scans) again after the
         // look-back, so they hold no registers while the next tile's loads are in flight
-        A tex[KR];
-        const A run = local_scan(x, tex);
+        L tex[KR];
+        const L run = local_scan(x, tex);
         if (lane == 0) s_w2[par][wid] = run;
         if (threadIdx.x == 0)
           s_next = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -183,9 +211,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
           if (wid != 0) load(xn, tn, 0);
         }
         if (wid == 0) {
-          A tile_tot = s_w2[par][0];
-#pragma unroll
-          for (int j = 1; j < NW; ++j) tile_tot = acc_add(tile_tot, s_w2[par][j]);
+          // wave totals scanned across lanes 0..NW-1 in the carry type
+          const A wt = lane < NW ? loc_to_acc<A>(s_w2[par][lane]) : acc_zero<A>();
+          const A wincl = row_incl_scan(wt);
+          const A tile_tot = read_lane(wincl, NW - 1);
           A prefix;
           if (tile == ti.first_tile) {
             prefix = EXCL ? init : acc_zero<A>();
@@ -195,12 +224,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
           }
           if (tile != ti.last_tile && lane == 0)
             publish<A>(status, tile, epoch, kStIncl, acc_add(prefix, tile_tot));
-          if (lane == 0) s_prefix = prefix;
+          const A wexcl = dpp_mov<kRowShr1, 0xf, 0xf>(wincl);
+          if (lane < NW) s_off[lane] = acc_add(prefix, wexcl);
           if (more) load(xn, tn, 0);
         }
         __syncthreads();
-        A w = s_prefix;
-        for (int j = 0; j < wid; ++j) w = acc_add(w, s_w2[par][j]);
+        const A w = s_off[wid];
         local_scan(x, tex);
         emit(x, tex, w, ti, 0);
         if (!more) break;
@@ -234,34 +263,34 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
 #pragma unroll
     for (int sub = 0; sub < S - 1; ++sub) {
       load(x, ti, sub);
-      A c = acc_zero<A>();
+      L c = acc_zero<L>();
 #pragma unroll
       for (int k = 0; k < KR; ++k) {
         V v;
 #pragma unroll
         for (int j = 0; j < VEC; ++j) {
           v.v[j] = x[k][j];
-          c = acc_add(c, to_acc(x[k][j]));
+          c = acc_add(c, to_loc(x[k][j]));
         }
         s_stash[(sub * KR + k) * BLOCK + threadIdx.x] = v;
       }
-      const A tot = read_lane(wave_incl_scan(c), kWave - 1);
+      const L tot = read_lane(wave_incl_scan(c), kWave - 1);
       if (lane == 0) s_w[sub][wid] = tot;
     }
 
     // ---- last sub-tile scanned in registers
     load(x, ti, S - 1);
-    A tex[KR];
-    const A run = local_scan(x, tex);
+    L tex[KR];
+    const L run = local_scan(x, tex);
     if (lane == 0) s_w[S - 1][wid] = run;
     __syncthreads();
 
     if (wid == 0) {
-      A tile_tot = s_w[0][0];
+      A tile_tot = loc_to_acc<A>(s_w[0][0]);
 #pragma unroll
       for (int sub = 0; sub < S; ++sub)
 #pragma unroll
-        for (int j = (sub == 0 ? 1 : 0); j < NW; ++j) tile_tot = acc_add(tile_tot, s_w[sub][j]);
+        for (int j = (sub == 0 ? 1 : 0); j < NW; ++j) tile_tot = acc_add_loc(tile_tot, s_w[sub][j]);
       A prefix;
       if (tile == ti.first_tile) {
         prefix = EXCL ? init : acc_zero<A>();
@@ -284,10 +313,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
       for (int sub = 0; sub < S; ++sub) {
         sub_base[sub] = b;
         A w = acc_zero<A>();
-        for (int j = 0; j < wid; ++j) w = acc_add(w, s_w[sub][j]);
+        for (int j = 0; j < wid; ++j) w = acc_add_loc(w, s_w[sub][j]);
         wave_off[sub] = w;
         if (sub + 1 < S)
-          for (int j = 0; j < NW; ++j) b = acc_add(b, s_w[sub][j]);
+          for (int j = 0; j < NW; ++j) b = acc_add_loc(b, s_w[sub][j]);
       }
     }
     emit(x, tex, acc_add(sub_base[S - 1], wave_off[S - 1]), ti, S - 1);

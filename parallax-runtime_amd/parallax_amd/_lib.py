@@ -107,7 +107,7 @@ _proto("pxb_plan_destroy", None, _vp)
 # ---- parallax/px_serve.h
 class PxsConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
-                ("max_batch", ctypes.c_int), ("max_wait_us", ctypes.c_double)]
+                ("max_batch", ctypes.c_int), ("max_wait_us", ctypes.c_double), ("lanes", ctypes.c_int)]
 
 
 class PxsRequest(ctypes.Structure):
