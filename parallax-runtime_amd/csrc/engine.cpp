@@ -469,7 +469,14 @@ pxk::TileCfg seg_tile_default(int op, int dtype, const SegSpec& sp) {
   pxk::TileCfg best{0, 0};
   for (const pxk::TileCfg& t : seg_tile_candidates(op, dtype)) {
     if (best.block == 0) best = t;  // smallest candidate first
-    if (pxk::tile_elems(dtype, t) <= mean && pxk::tile_elems(dtype, t) <= pxk::tile_elems(dtype, big)) best = t;
+    if (op == PX_OP_REDUCE) {
+      // reduce tiles sync once per tile and leave one partial each: long segments take
+      // the largest tile of at most a quarter of the mean length (served f64 reduces of
+      // ~2M elements: 256 x 16)
+      if (pxk::tile_elems(dtype, t) <= std::max<uint64_t>(mean / 4, pxk::tile_elems(dtype, big))) best = t;
+    } else if (pxk::tile_elems(dtype, t) <= mean && pxk::tile_elems(dtype, t) <= pxk::tile_elems(dtype, big)) {
+      best = t;
+    }
   }
   return best;
 }

@@ -111,10 +111,20 @@ __device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t state, 
 }
 
 // ---------------------------------------------------------------- histograms
-// Adds N encoded keys per lane to a wave's LDS histograms hw[D * 256].  A digit position
-// that all N keys of every active lane share is added once, by the first active lane (64
-// lanes adding to one LDS word serialise; narrow keys have such digits in most waves);
-// the test costs one OR of differences per key and one ballot per digit position.
+// Lanes that must share the first lane's digit before hist_add counts a digit position
+// by ballots (random 8-bit digits: 1 in 4 lanes on average).
+constexpr int kSkewLanes = 8;
+
+// Adds N encoded keys per lane to a wave's LDS histograms hw[D * 256].  Lanes adding to
+// one LDS word serialise, so:
+//  * a digit position that all N keys of every active lane share is added once, by the
+//    first active lane (narrow keys have such digits in most waves); the test costs one
+//    OR of differences per key and one ballot per digit position;
+//  * a skewed position (the first lane's digit of key 0 shared by kSkewLanes lanes or
+//    more: small integers' upper bytes, float exponents) counts its two most common
+//    digits (the first lane's, and the first other lane's) by ballots, and only the
+//    lanes holding another digit add to LDS;
+//  * other positions add every key's digit.
 template <int D, int N, class U>
 __device__ __forceinline__ void hist_add(uint32_t* hw, const U (&k)[N], int lane) {
   U ref;  // the first active lane's first key (readfirstlane returns int: each half is
@@ -130,12 +140,35 @@ __device__ __forceinline__ void hist_add(uint32_t* hw, const U (&k)[N], int lane
 #pragma unroll
   for (int j = 1; j < N; ++j) diff |= k[j] ^ ref;
   const uint64_t active = __ballot(1);
+  const bool leader = (active & ((1ull << lane) - 1ull)) == 0ull;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
+    const uint32_t r1 = static_cast<uint32_t>(ref >> (8 * d)) & 0xffu;
     if (__ballot(((diff >> (8 * d)) & 0xffu) != 0u) == 0ull) {
-      if ((active & ((1ull << lane) - 1ull)) == 0ull)
-        atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(ref >> (8 * d)) & 0xffu)],
-                  static_cast<uint32_t>(N * __popcll(active)));
+      if (leader) atomicAdd(&hw[d * kBins + r1], static_cast<uint32_t>(N * __popcll(active)));
+      continue;
+    }
+    const uint32_t d0 = static_cast<uint32_t>(k[0] >> (8 * d)) & 0xffu;
+    const uint64_t m1 = __ballot(d0 == r1);
+    if (__popcll(m1) >= kSkewLanes) {
+      const uint64_t rest = active & ~m1;
+      const uint32_t r2 = rest ? static_cast<uint32_t>(__builtin_amdgcn_readlane(
+                                     static_cast<int>(d0), __builtin_ctzll(rest)))
+                               : r1;
+      uint32_t c1 = 0, c2 = 0;  // wave-uniform counts of r1 and r2
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const uint32_t dj = static_cast<uint32_t>(k[j] >> (8 * d)) & 0xffu;
+        const bool e1 = dj == r1;
+        const bool e2 = !e1 && dj == r2;
+        c1 += static_cast<uint32_t>(__popcll(__ballot(e1)));
+        c2 += static_cast<uint32_t>(__popcll(__ballot(e2)));
+        if (!e1 && !e2) atomicAdd(&hw[d * kBins + dj], 1u);
+      }
+      if (leader) {
+        atomicAdd(&hw[d * kBins + r1], c1);
+        if (c2) atomicAdd(&hw[d * kBins + r2], c2);
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k[j] >> (8 * d)) & 0xffu)], 1u);
@@ -814,7 +847,12 @@ hipError_t seg_radix_sort_t(void* data_v, const RadixSegs& g, const Scratch& s,
   k_fill_u32<<<static_cast<uint32_t>(std::min<uint64_t>((ncnt + 255) / 256, 1024)), 256, 0, stream>>>(g.seg_cnt, ncnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint64_t hist_per_cu = tune().seg_hist_bpc > 0 ? static_cast<uint64_t>(tune().seg_hist_bpc) : 4;
+  // Counting blocks per CU: each adds its counts to a segment's totals with device-scope
+  // atomics whenever its tiles change segment, so few long segments count best on 2 (a
+  // served sort group of ~30 segments: 3.40 vs 3.48 ms for the group) and many on 4
+  // (4,096 segments of 16..2^18 keys: 1.83 vs 1.88 ms).
+  const uint64_t hist_per_cu = tune().seg_hist_bpc > 0 ? static_cast<uint64_t>(tune().seg_hist_bpc)
+                                                       : (g.nseg <= 128 ? 2 : 4);
   const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, hist_per_cu * cus));
   k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, g, s.ctl);
   if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -211,9 +211,17 @@ __global__ __launch_bounds__(256) void k_seg_combine(SegGeom g,
   const uint32_t first = g.tile_off[seg];
   const uint32_t last = g.tile_off[seg + 1] - 1;
   if (last == first) return;  // written by pass 1
-  A c = acc_zero<A>();
-  for (uint32_t j = first + lane_id(); j <= last; j += kWave) c = acc_add(c, partials[j]);
-  const A w = wave_sum(c);
+  // four independent chains, so four partials per lane are in flight at once
+  A c0 = acc_zero<A>(), c1 = acc_zero<A>(), c2 = acc_zero<A>(), c3 = acc_zero<A>();
+  uint32_t j = first + lane_id();
+  for (; j + 3 * kWave <= last; j += 4 * kWave) {
+    c0 = acc_add(c0, partials[j]);
+    c1 = acc_add(c1, partials[j + kWave]);
+    c2 = acc_add(c2, partials[j + 2 * kWave]);
+    c3 = acc_add(c3, partials[j + 3 * kWave]);
+  }
+  for (; j <= last; j += kWave) c0 = acc_add(c0, partials[j]);
+  const A w = wave_sum(acc_add(acc_add(c0, c1), acc_add(c2, c3)));
   if (lane_id() == 0) out[g.seg_out ? g.seg_out[seg] : seg] = from_acc<T>(w);
 }
 

@@ -108,6 +108,10 @@ size_t sort_scratch(int dtype, uint64_t lo, uint64_t n) {
 
 enum LaunchResult { kLaunched, kNoMemory, kFailed };
 
+// Served reduces: 256 threads x 16 vectors per tile (a request up to one tile long is
+// summed by one block; longer ones leave one partial per 16 KiB of input).
+constexpr pxk::TileCfg kServedReduceTile{256, 16};
+
 }  // namespace
 
 struct pxs_server {
@@ -228,13 +232,17 @@ int pxs_server::launch(Slot& s, const std::vector<int64_t>& ids) {
     }
     s.pin_cap = nseg;
   }
+  // Tile shapes fixed per (op, dtype): a request's float arithmetic (the order its
+  // elements are summed in) must not depend on what it is batched with, so its result is
+  // the same alone and in any mix.
   int rc = PX_EOK;
   switch (r0.r.op) {
-    case PX_OP_REDUCE: rc = parallax::run_seg_reduce(*s.ctx, r0.r.dtype, b, sp, b); break;
-    case PX_OP_SCAN: rc = parallax::run_seg_scan(*s.ctx, r0.r.dtype, b, b, sp); break;
+    case PX_OP_REDUCE: rc = parallax::run_seg_reduce(*s.ctx, r0.r.dtype, b, sp, b, kServedReduceTile); break;
+    case PX_OP_SCAN: rc = parallax::run_seg_scan(*s.ctx, r0.r.dtype, b, b, sp, pxk::tune().scan[r0.r.dtype]); break;
     case PX_OP_SORT: rc = parallax::run_seg_sort(*s.ctx, r0.r.dtype, b, sp, s.scratch, scratch_bytes); break;
     case PX_OP_FLAGS:
-      rc = parallax::run_seg_compact(*s.ctx, r0.r.flags, b, b, sp, static_cast<uint64_t*>(s.counts));
+      rc = parallax::run_seg_compact(*s.ctx, r0.r.flags, b, b, sp, static_cast<uint64_t*>(s.counts),
+                                     pxk::tune().compact[r0.r.flags.dtype]);
       if (rc == PX_EOK &&
           hipMemcpyAsync(s.pin_counts, s.counts, nseg * sizeof(uint64_t), hipMemcpyDeviceToHost,
                          s.ctx->stream()) != hipSuccess)
