@@ -168,23 +168,29 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
 
   for (uint32_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const TileInfo ti = tile_info<VEC, TILE>(g, tile, shift);
-    A acc = acc_zero<A>();
+    // four accumulator chains (vector k adds into chain k % 4), folded in a fixed order
+    A acc[4] = {acc_zero<A>(), acc_zero<A>(), acc_zero<A>(), acc_zero<A>()};
+    if (ti.w0 >= ti.lo && ti.w0 + TILE <= ti.hi) {  // whole tile inside the segment
+      V v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t p0 = ti.w0 + (static_cast<int64_t>(k) * 256 + threadIdx.x) * VEC;
-      if (p0 < ti.hi && p0 + VEC > ti.lo) {
-        const V v = vin[p0 / VEC];
-        if (p0 >= ti.lo && p0 + VEC <= ti.hi) {
+      for (int k = 0; k < K; ++k) v[k] = ld_stream(vin + ti.w0 / VEC + k * 256 + threadIdx.x);
 #pragma unroll
-          for (int j = 0; j < VEC; ++j) acc = acc_add(acc, to_acc(v.v[j]));
-        } else {
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) acc[k & 3] = acc_add(acc[k & 3], to_acc(v[k].v[j]));
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int64_t p0 = ti.w0 + (static_cast<int64_t>(k) * 256 + threadIdx.x) * VEC;
+        if (p0 < ti.hi && p0 + VEC > ti.lo) {
+          const V v = vin[p0 / VEC];
 #pragma unroll
           for (int j = 0; j < VEC; ++j)
-            if (p0 + j >= ti.lo && p0 + j < ti.hi) acc = acc_add(acc, to_acc(v.v[j]));
+            if (p0 + j >= ti.lo && p0 + j < ti.hi) acc[k & 3] = acc_add(acc[k & 3], to_acc(v.v[j]));
         }
       }
     }
-    const A w = wave_sum(acc);
+    const A w = wave_sum(acc_add(acc_add(acc[0], acc[1]), acc_add(acc[2], acc[3])));
     if (lane == 0) s_w[wid] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
