@@ -1,8 +1,8 @@
 """Test configuration: the `gpu` marker, import paths, and shared helpers.
 
-CPU tests (-m "not gpu") check the oracle against known answers and the reference's own
-fallback, the data generators, and that the C ABI library loads and exports every symbol
-the public headers declare.  GPU tests (-m gpu) call the HIP engine through its C ABI and
+CPU tests (-m "not gpu") check the oracle against the reference's known answers, closed
+forms and brute force, the data generators, and that the C ABI library loads and exports
+every symbol the public headers declare.  GPU tests (-m gpu) call the HIP engine through its C ABI and
 compare with the oracle on the same inputs.
 """
 import os

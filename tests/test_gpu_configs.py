@@ -230,7 +230,7 @@ def _fetch(server, rid, op, a):
 def test_served_mixed_stream_every_result(gpu_ctx):
     reqs = datagen.served_requests(seed=4, nreq=400)
     cfg = PxsConfig()
-    lib.pxs_config_default(ctypes.byref(cfg))  # 64 GB pool, 4 slots
+    lib.pxs_config_default(ctypes.byref(cfg))  # 64 GB pool, 8 slots on one stream
     server = lib.pxs_create(ctypes.byref(cfg))
     assert server
     ptrs = []
