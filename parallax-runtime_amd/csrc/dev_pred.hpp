@@ -57,16 +57,36 @@ __device__ __forceinline__ bool pred_eval(uint32_t fn, uint32_t flags, T c0, T c
 }
 
 // Predicate with its constants unpacked once per kernel.  MOD_EQ by a positive power of
-// two uses a mask: for x < 0 the truncated remainder is (x & (m-1)) - m when nonzero.
+// two m is one AND and one compare, (x & am) == ac: the truncated remainder of x is
+// x & (m-1) for x >= 0 and (x & (m-1)) - m for x < 0 when nonzero, so
+//   c == 0:          am = m-1,            ac = 0 (any sign);
+//   0 < c < m:       am = (m-1) | sign,   ac = c (x >= 0);
+//   -m < c < 0:      am = (m-1) | sign,   ac = sign | (c + m) (x < 0);
+//   |c| >= m:        never (am = 0, ac = 1).
 template <class T>
 struct Pred {
   uint32_t fn, flags;
   T c0, c1;
   T mod_mask;  // m - 1 when MOD_EQ's divisor m is a positive power of two, else 0
+  T am, ac;    // the AND mask and compare value of that case
   __host__ __device__ __forceinline__ Pred(const px_desc& d)
-      : fn(d.fn), flags(d.flags), c0(bits_as<T>(d.c0)), c1(bits_as<T>(d.c1)), mod_mask(T(0)) {
+      : fn(d.fn), flags(d.flags), c0(bits_as<T>(d.c0)), c1(bits_as<T>(d.c1)), mod_mask(T(0)), am(T(0)), ac(T(1)) {
     if constexpr (std::is_integral_v<T>) {
-      if (fn == PX_PRED_MOD_EQ && c0 > T(0) && (c0 & (c0 - T(1))) == T(0)) mod_mask = c0 - T(1);
+      if (fn == PX_PRED_MOD_EQ && c0 > T(0) && (c0 & (c0 - T(1))) == T(0)) {
+        mod_mask = c0 - T(1);
+        using U = std::make_unsigned_t<T>;
+        const U mm = static_cast<U>(mod_mask), sign = static_cast<U>(U(1) << (8 * sizeof(T) - 1));
+        if (c1 == T(0)) {
+          am = static_cast<T>(mm);
+          ac = T(0);
+        } else if (c1 > T(0) && c1 <= mod_mask) {
+          am = static_cast<T>(std::is_signed_v<T> ? (mm | sign) : mm);
+          ac = c1;
+        } else if (std::is_signed_v<T> && c1 < T(0) && c1 >= static_cast<T>(-mod_mask)) {
+          am = static_cast<T>(mm | sign);
+          ac = static_cast<T>(sign | static_cast<U>(c1 + c0));
+        }
+      }
     }
   }
   __device__ __forceinline__ bool operator()(T x) const { return pred_eval<T>(fn, flags, c0, c1, x); }
@@ -95,12 +115,8 @@ struct Pred {
       case PX_PRED_MOD_EQ:
         if constexpr (std::is_integral_v<T>) {
           if (mod_mask != T(0)) {
-            const T mm = mod_mask;
-            if constexpr (std::is_signed_v<T>) {
-              PXK_PMASK(((v & mm) != T(0) && v < T(0) ? T((v & mm) - mm - T(1)) : T(v & mm)) == c1)
-            } else {
-              PXK_PMASK(T(v & mm) == c1)
-            }
+            const T a = am, c = ac;
+            PXK_PMASK(T(v & a) == c)
             break;
           }
         }

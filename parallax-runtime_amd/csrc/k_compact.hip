@@ -31,6 +31,9 @@ namespace {
 
 // Writes the staged run src[0, count) to dst (any element alignment): a scalar head up
 // to the next 16-byte boundary, 16-byte non-temporal stores for the body, a scalar tail.
+// `src` is 16-byte aligned and readable one vector past `count` (the staging area's
+// spare vector): a body vector is read as the two aligned LDS vectors around it (16-byte
+// reads, no bank conflicts) and shifted by the run's alignment, uniform per run.
 template <class T, int BLOCK, class C>
 __device__ __forceinline__ void write_run(T* dst, const T* src, C count) {
   constexpr int VEC = 16 / sizeof(T);
@@ -41,12 +44,28 @@ __device__ __forceinline__ void write_run(T* dst, const T* src, C count) {
   if (threadIdx.x < head) dst[threadIdx.x] = src[threadIdx.x];
   const uint32_t nvec = static_cast<uint32_t>((count - head) / VEC);
   V* vd = reinterpret_cast<V*>(dst + head);
+  const V* vs = reinterpret_cast<const V*>(src);
+  // the body starts `head` (< VEC, uniform) elements past an LDS vector
+  auto body = [&](auto shc) {
+    constexpr int SH = decltype(shc)::value;
 #pragma unroll 1
-  for (uint32_t v = threadIdx.x; v < nvec; v += BLOCK) {
-    V y;
+    for (uint32_t v = threadIdx.x; v < nvec; v += BLOCK) {
+      if constexpr (SH == 0) {
+        st_stream(vd + v, vs[v]);
+      } else {
+        const V lo = vs[v], hi = vs[v + 1];
+        V y;
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) y.v[j] = src[head + v * VEC + j];
-    st_stream(vd + v, y);
+        for (int j = 0; j < VEC; ++j) y.v[j] = (j + SH < VEC) ? lo.v[j + SH] : hi.v[j + SH - VEC];
+        st_stream(vd + v, y);
+      }
+    }
+  };
+  switch (head) {
+    case 0: body(std::integral_constant<int, 0>{}); break;
+    case 1: body(std::integral_constant<int, 1>{}); break;
+    case 2: if constexpr (VEC > 2) body(std::integral_constant<int, 2>{}); break;
+    default: if constexpr (VEC > 3) body(std::integral_constant<int, 3>{}); break;
   }
   const uint32_t done = head + nvec * VEC;
   if (threadIdx.x < count - done) dst[done + threadIdx.x] = src[done + threadIdx.x];
@@ -71,7 +90,10 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
 
-  __shared__ S s_stage[TILE];
+  // the staged run, one spare vector (write_run reads one past the run) and, for the
+  // kept-only mode, one slot per thread that rejected elements are written to
+  constexpr int kSpare = 16 / sizeof(S) + (MODE == kModeKept ? BLOCK : 0);
+  __shared__ __attribute__((aligned(16))) S s_stage[TILE + kSpare];
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_next;
   __shared__ uint32_t s_epoch;
@@ -203,6 +225,22 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
 
   // Stages the kept (and, for partition, rejected) elements in LDS, in order.
   auto stage = [&](const Tile& u) {
+    if constexpr (MODE == kModeKept) {
+      // branch-free: every element is written, a rejected one to this thread's slot
+      const C dummy = static_cast<C>(TILE + 16 / sizeof(S) + threadIdx.x);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        C r = u.wave_off + u.tex[k];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const int e = k * VEC + j;
+          const C f = static_cast<C>((u.flags >> e) & 1u);
+          s_stage[f ? r : dummy] = u.x[e];
+          r += f;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       C r = u.wave_off + u.tex[k];

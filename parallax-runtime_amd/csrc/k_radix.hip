@@ -417,6 +417,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   const int wid = threadIdx.x / kWave;
   const uint32_t* bins = ctl->radix_bins + (shift / 8) * kBins;
   const uint64_t lanes_below = (1ull << lane) - 1ull;
+  // SEG: the bits of this digit that vary across the batch's long segments; a full tile
+  // (no pad keys, which have every bit set) ranks on those bits only
+  const uint32_t vary = SEG ? static_cast<uint32_t>(ld_agent(&ctl->radix_vary) >> shift) & 0xffu : 0xffu;
 
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
 
@@ -471,33 +474,37 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // ---- rank among equal digits of the wave, in input order (two 16-bit ranks per
     // register: a tile holds fewer than 2^16 keys)
     uint32_t rank2[(KPT + 1) / 2] = {};
+    auto rank_keys = [&](auto masked) {
 #pragma unroll
-    for (int j = 0; j < KPT; ++j) {
-      const uint32_t d = digit_of(key[j], shift);
-      // Lanes whose digit differs from this lane's: some bit's ballot disagrees with
-      // this lane's bit (sign-extended to a full word, the same for both halves).
-      uint32_t own[8];
-      uint64_t m[8];
+      for (int j = 0; j < KPT; ++j) {
+        const uint32_t d = digit_of(key[j], shift);
+        // Lanes whose digit differs from this lane's: some bit's ballot disagrees with
+        // this lane's bit (sign-extended to a full word, the same for both halves).
+        // Masked: a bit every key of the batch shares (all 64 lanes hold real keys)
+        // agrees in every lane and is skipped (a uniform branch).
+        uint32_t diff_lo = 0, diff_hi = 0;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        own[b] = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int32_t>(d), b, 1));
-        asm volatile("" : "+v"(own[b]));  // ballot from the extracted bit: one compare
-        m[b] = __ballot(own[b] != 0u);
+        for (int b = 0; b < 8; ++b) {
+          if (decltype(masked)::value && !((vary >> b) & 1u)) continue;
+          uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int32_t>(d), b, 1));
+          asm volatile("" : "+v"(own));  // ballot from the extracted bit: one compare
+          const uint64_t m = __ballot(own != 0u);
+          // diff |= m ^ own in one three-input op per half (truth table 0xF6 for
+          // S0 | (S1 ^ S2) with S0, S1, S2 = 0xF0, 0xCC, 0xAA).
+          diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, static_cast<uint32_t>(m), own, 0xF6);
+          diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, static_cast<uint32_t>(m >> 32), own, 0xF6);
+        }
+        const uint64_t peers = ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
+        const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below));
+        const uint32_t c = s_wh[wid][d];
+        rank2[j / 2] |= (c + below) << (16 * (j & 1));
+        if (below == 0) s_wh[wid][d] = c + static_cast<uint32_t>(__popcll(peers));
       }
-      // diff |= m ^ own in one three-input op per half (truth table 0xF6 for
-      // S0 | (S1 ^ S2) with S0, S1, S2 = 0xF0, 0xCC, 0xAA).
-      uint32_t diff_lo = 0, diff_hi = 0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, static_cast<uint32_t>(m[b]), own[b], 0xF6);
-        diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, static_cast<uint32_t>(m[b] >> 32), own[b], 0xF6);
-      }
-      const uint64_t peers = ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
-      const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below));
-      const uint32_t c = s_wh[wid][d];
-      rank2[j / 2] |= (c + below) << (16 * (j & 1));
-      if (below == 0) s_wh[wid][d] = c + static_cast<uint32_t>(__popcll(peers));
-    }
+    };
+    if (SEG && vary != 0xffu && tend - tbase == static_cast<uint32_t>(TILE))
+      rank_keys(std::true_type{});
+    else
+      rank_keys(std::false_type{});
     __syncthreads();
 
     // ---- per digit: wave offsets, tile count (published), block-wide exclusive scan
@@ -827,6 +834,7 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
   for (int d = 0; d < D; ++d)
     if (((varying >> (8 * d)) & 0xffu) == 0u) skip |= 1u << d;
   st_agent32(&ctl->radix_skip, skip);
+  st_agent(&ctl->radix_vary, varying);
   st_agent32(&ctl->done, 0u);
 }
 
