@@ -8,8 +8,10 @@
  * its own scratch) frees up are fused into ONE segmented launch per (op, dtype,
  * predicate) group, and batches are queued on the device while earlier ones run and new
  * requests keep arriving (continuous batching).  Slots share `lanes` streams (default
- * one): batches of one lane run back to back in launch order, so the device never waits
- * for the host between them and two whole-device look-back kernels never share the CUs.
+ * two): batches of one lane run back to back in launch order, so the device never waits
+ * for the host between them; scans, compactions and sorts (whole-device look-back
+ * kernels, which must not share the CUs with each other) take the first lane and
+ * reduces the last.
  * A request's result does not depend on what it was batched with: each request is one
  * segment with the same tiling it would get alone.
  *
@@ -41,7 +43,8 @@ typedef struct pxs_config {
   size_t hbm_budget;    /* bytes of the device pool (inputs + outputs); default 64 GiB */
   int max_batch;        /* most requests fused into one launch; default 4096 */
   double max_wait_us;   /* how long a lone request may wait for batch-mates; default 0 */
-  int lanes;            /* streams the slots share (1..slots); default 1 */
+  int lanes;            /* streams the slots share (1..slots); default 2: reduce batches
+                           on the last one, every other batch on the first */
 } pxs_config;
 
 typedef struct pxs_request {

@@ -78,11 +78,28 @@ struct RadixKey<double> {
   }
 };
 
-// The key whose order-preserving bits are all ones (every digit 255): pads positions
-// past n, which then sort after every real key of their tile.
 template <class T>
-__device__ __forceinline__ T radix_pad() {
-  if constexpr (sizeof(T) == 4) {
+constexpr bool kSignedKey = std::is_integral_v<T> && std::is_signed_v<T>;
+
+// The key a sort orders by.  Signed integers key on (U)x - base with base = (U)min(x)
+// (k_radix_min): order-preserving on [min, max], and small values of either sign then
+// share their upper digits, whose passes are skipped (int32 in [-1000, 1000]: 2 of 4
+// passes); base = (U)INT_MIN would be the sign-bit flip, at the same one VALU op.  Other
+// types key on their order-preserving encoding (`base` unused).
+template <class T>
+__device__ __forceinline__ typename RadixKey<T>::U key_of(T x, typename RadixKey<T>::U base) {
+  using U = typename RadixKey<T>::U;
+  if constexpr (kSignedKey<T>) return static_cast<U>(static_cast<U>(x) - base);
+  else return RadixKey<T>::enc(x);
+}
+
+// The element whose key is all ones (every digit 255): pads positions past n, which
+// then sort after every real key of their tile.
+template <class T>
+__device__ __forceinline__ T radix_pad(typename RadixKey<T>::U base) {
+  if constexpr (kSignedKey<T>) {
+    return static_cast<T>(base - 1u);
+  } else if constexpr (sizeof(T) == 4) {
     const uint32_t u = std::is_same_v<T, uint32_t> ? 0xffffffffu : 0x7fffffffu;
     return __builtin_bit_cast(T, u);
   } else {
@@ -92,8 +109,21 @@ __device__ __forceinline__ T radix_pad() {
 }
 
 template <class T>
-__device__ __forceinline__ uint32_t digit_of(T x, int shift) {
-  return static_cast<uint32_t>(RadixKey<T>::enc(x) >> shift) & 0xffu;
+__device__ __forceinline__ uint32_t digit_of(T x, int shift, typename RadixKey<T>::U base) {
+  return static_cast<uint32_t>(key_of(x, base) >> shift) & 0xffu;
+}
+
+// The key base the counting kernels read for a signed sort: (U)min(x) from ctl->radix_nmin
+// (the complement of the smallest encoded key); 0 for other types.
+template <class T>
+__device__ __forceinline__ typename RadixKey<T>::U read_key_base(const Ctl* ctl) {
+  using U = typename RadixKey<T>::U;
+  if constexpr (kSignedKey<T>) {
+    constexpr U sign = static_cast<U>(U(1) << (8 * sizeof(U) - 1));
+    return static_cast<U>(static_cast<U>(~ld_agent(&ctl->radix_nmin)) ^ sign);
+  } else {
+    return U(0);
+  }
 }
 
 // Keys per lane of a whole-array digit pass: 16 for 8-byte keys (1024-thread tiles of 16K
@@ -193,8 +223,9 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   uint32_t* hw = h[wid];
   for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0;  // the wave's own copy
   using U = typename RadixKey<T>::U;
+  const U base = read_key_base<T>(ctl);
   auto count = [&](T x) {
-    const U k = RadixKey<T>::enc(x);
+    const U k = key_of(x, base);
 #pragma unroll
     for (int d = 0; d < D; ++d) atomicAdd(&hw[d * kBins + (static_cast<uint32_t>(k >> (8 * d)) & 0xffu)], 1u);
   };
@@ -227,7 +258,7 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
 #pragma unroll
     for (int u = 0; u < UL; ++u)
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) k[u * VEC + j] = RadixKey<T>::enc(cur[u].v[j]);
+      for (int j = 0; j < VEC; ++j) k[u * VEC + j] = key_of(cur[u].v[j], base);
     hist_add<D>(hw, k, lane);
 #pragma unroll
     for (int u = 0; u < UL; ++u) cur[u] = nxt[u];
@@ -238,7 +269,7 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
     const V x = ld_stream(vp + v);
     U k[VEC];
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) k[j] = RadixKey<T>::enc(x.v[j]);
+    for (int j = 0; j < VEC; ++j) k[j] = key_of(x.v[j], base);
     hist_add<D>(hw, k, lane);
   }
   const uint64_t body_end = head + nv * VEC;
@@ -281,8 +312,57 @@ __global__ __launch_bounds__(HB) void k_radix_hist(const T* data, uint64_t n, Ct
   }
   if (threadIdx.x == 0) {
     st_agent32(&ctl->radix_skip, skip);
+    if constexpr (kSignedKey<T>) {  // every block has read the minimum
+      st_agent(&ctl->radix_base, static_cast<uint64_t>(base));
+      st_agent(&ctl->radix_nmin, 0ull);
+    }
     st_agent32(&ctl->done, 0u);
   }
+}
+
+// Signed integer sorts: the minimum of the array (its key base).  Every block adds the
+// complement of its smallest encoded key to ctl->radix_nmin with an agent-scope max (the
+// slot is zero between calls: the histogram kernel that reads it resets it).
+template <class T>
+__device__ __forceinline__ void radix_min_publish(uint64_t nm, Ctl* ctl) {
+  __shared__ uint64_t s_nm[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(static_cast<unsigned long long>(nm), off);
+    nm = o > nm ? o : nm;
+  }
+  if (lane_id() == 0) s_nm[threadIdx.x / kWave] = nm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) nm = s_nm[w] > nm ? s_nm[w] : nm;
+    __hip_atomic_fetch_max(&ctl->radix_nmin, nm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_radix_min(const T* data, uint64_t n, Ctl* ctl) {
+  using U = typename RadixKey<T>::U;
+  constexpr int VEC = 16 / sizeof(T);
+  using V = Vec<T, VEC>;
+  uint64_t nm = 0;  // max of ~enc(x)
+  auto add = [&](T x) {
+    const uint64_t c = static_cast<U>(~RadixKey<T>::enc(x));
+    nm = c > nm ? c : nm;
+  };
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * 256;
+  const uint32_t mis = static_cast<uint32_t>((reinterpret_cast<uintptr_t>(data) & 15u) / sizeof(T));
+  const uint64_t head = (mis ? VEC - mis : 0u) < n ? (mis ? VEC - mis : 0u) : n;
+  if (gtid < head) add(data[gtid]);
+  const V* vp = reinterpret_cast<const V*>(data + head);
+  const uint64_t nv = (n - head) / VEC;
+  for (uint64_t v = gtid; v < nv; v += gstride) {
+    const V x = ld_stream(vp + v);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) add(x.v[j]);
+  }
+  const uint64_t body_end = head + nv * VEC;
+  if (gtid < n - body_end) add(data[body_end + gtid]);
+  radix_min_publish<T>(nm, ctl);
 }
 
 // ---------------------------------------------------------------- segment geometry
@@ -420,6 +500,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   // SEG: the bits of this digit that vary across the batch's long segments; a full tile
   // (no pad keys, which have every bit set) ranks on those bits only
   const uint32_t vary = SEG ? static_cast<uint32_t>(ld_agent(&ctl->radix_vary) >> shift) & 0xffu : 0xffu;
+  using UK = typename RadixKey<T>::U;
+  const UK base = kSignedKey<T> ? static_cast<UK>(ld_agent(&ctl->radix_base)) : UK(0);
 
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
 
@@ -469,7 +551,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     T key[KPT];
     const T* lp = tin + first;
 #pragma unroll
-    for (int j = 0; j < KPT; ++j) key[j] = j < nv ? __builtin_nontemporal_load(lp + j * kWave) : radix_pad<T>();
+    for (int j = 0; j < KPT; ++j) key[j] = j < nv ? __builtin_nontemporal_load(lp + j * kWave) : radix_pad<T>(base);
 
     // ---- rank among equal digits of the wave, in input order (two 16-bit ranks per
     // register: a tile holds fewer than 2^16 keys)
@@ -477,7 +559,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     auto rank_keys = [&](auto masked) {
 #pragma unroll
       for (int j = 0; j < KPT; ++j) {
-        const uint32_t d = digit_of(key[j], shift);
+        const uint32_t d = digit_of(key[j], shift, base);
         // Lanes whose digit differs from this lane's: some bit's ballot disagrees with
         // this lane's bit (sign-extended to a full word, the same for both halves).
         // Masked: a bit every key of the batch shares (all 64 lanes hold real keys)
@@ -547,7 +629,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // runs before the look-back and gives the predecessors time to publish
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
-      s_keys[s_wh[wid][digit_of(key[j], shift)] + ((rank2[j / 2] >> (16 * (j & 1))) & 0xffffu)] = key[j];
+      s_keys[s_wh[wid][digit_of(key[j], shift, base)] + ((rank2[j / 2] >> (16 * (j & 1))) & 0xffffu)] = key[j];
     }
 
     // ---- digit counts before this tile (windowed look-back), output offsets
@@ -602,12 +684,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
       for (int i0 = 0; i0 < KPT; ++i0) {
         const uint32_t i = static_cast<uint32_t>(i0) * BLOCK + threadIdx.x;
         const T k = s_keys[i];
-        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
+        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift, base)] + i)] = k;
       }
     } else {
       for (uint32_t i = threadIdx.x; i < tn; i += BLOCK) {
         const T k = s_keys[i];
-        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift)] + i)] = k;
+        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift, base)] + i)] = k;
       }
     }
     __syncthreads();
@@ -673,6 +755,11 @@ hipError_t radix_sort_t(void* data_v, uint64_t n, const Scratch& s, hipStream_t 
   const uint64_t hist_work = (n + HB * 16ull * (16 / sizeof(T)) - 1) / (HB * 16ull * (16 / sizeof(T)));
   const uint32_t hist_grid = static_cast<uint32_t>(std::min<uint64_t>(hist_per_cu * cus, hist_work ? hist_work : 1));
   hook("sort_hist", 0);
+  if constexpr (kSignedKey<T>) {
+    k_radix_min<T><<<std::min<uint32_t>(cus * 4, static_cast<uint32_t>((n + 4095) / 4096)), 256, 0, stream>>>(data, n, s.ctl);
+    const hipError_t e0 = hipGetLastError();
+    if (e0 != hipSuccess) return e0;
+  }
   if (HB == 1024)
     k_radix_hist<T, 1024><<<hist_grid, 1024, 0, stream>>>(data, n, s.ctl);
   else if (HB == 512)
@@ -730,8 +817,9 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
   for (int i = lane; i < D * kBins; i += kWave) hw[i] = 0u;  // the wave's own copy
   __syncthreads();
   U acc_or = 0, acc_nor = 0;
+  const U base = read_key_base<T>(ctl);
   auto count1 = [&](T x) {
-    const U k = RadixKey<T>::enc(x);
+    const U k = key_of(x, base);
     acc_or |= k;
     acc_nor |= static_cast<U>(~k);
 #pragma unroll
@@ -789,7 +877,7 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
 #pragma unroll
         for (int u = 0; u < UL; ++u)
 #pragma unroll
-          for (int j = 0; j < VEC; ++j) k[u * VEC + j] = RadixKey<T>::enc(x[u].v[j]);
+          for (int j = 0; j < VEC; ++j) k[u * VEC + j] = key_of(x[u].v[j], base);
         count_keys(k);
       } else {
 #pragma unroll
@@ -797,7 +885,7 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
           if (v0 + u * HB < nvec) {
             U k[VEC];
 #pragma unroll
-            for (int j = 0; j < VEC; ++j) k[j] = RadixKey<T>::enc(x[u].v[j]);
+            for (int j = 0; j < VEC; ++j) k[j] = key_of(x[u].v[j], base);
             count_keys(k);
           }
         }
@@ -835,7 +923,29 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
     if (((varying >> (8 * d)) & 0xffu) == 0u) skip |= 1u << d;
   st_agent32(&ctl->radix_skip, skip);
   st_agent(&ctl->radix_vary, varying);
+  if constexpr (kSignedKey<T>) {  // every block has read the minimum
+    st_agent(&ctl->radix_base, static_cast<uint64_t>(base));
+    st_agent(&ctl->radix_nmin, 0ull);
+  }
   st_agent32(&ctl->done, 0u);
+}
+
+// Signed integer keys of a batched sort: the minimum over every long segment (one key
+// base for the batch), as k_radix_min.
+template <class T>
+__global__ __launch_bounds__(256) void k_segradix_min(const T* data_all, RadixSegs g, Ctl* ctl) {
+  using U = typename RadixKey<T>::U;
+  uint64_t nm = 0;
+  for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    uint32_t sg, tb, te;
+    seg_tile<T>(g, t, sg, tb, te);
+    const T* data = data_all + g.seg_lo[sg];
+    for (uint32_t p = tb + threadIdx.x; p < te; p += 256) {
+      const uint64_t c = static_cast<U>(~RadixKey<T>::enc(__builtin_nontemporal_load(data + p)));
+      nm = c > nm ? c : nm;
+    }
+  }
+  radix_min_publish<T>(nm, ctl);
 }
 
 template <class T>
@@ -862,6 +972,10 @@ hipError_t seg_radix_sort_t(void* data_v, const RadixSegs& g, const Scratch& s,
   const uint64_t hist_per_cu = tune().seg_hist_bpc > 0 ? static_cast<uint64_t>(tune().seg_hist_bpc)
                                                        : (g.nseg <= 128 ? 2 : 4);
   const uint32_t hgrid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, hist_per_cu * cus));
+  if constexpr (kSignedKey<T>) {
+    k_segradix_min<T><<<static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, 4ull * cus)), 256, 0, stream>>>(data, g, s.ctl);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, g, s.ctl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const bool large = g.tile == static_cast<uint32_t>(kSegBlockLarge * kKptSeg<T>);

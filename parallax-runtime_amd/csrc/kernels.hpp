@@ -26,7 +26,9 @@ struct Ctl {
   uint32_t pad0;
   uint64_t result;  // kept count written by the last compaction tile
   uint64_t radix_vary;  // segmented radix: key bits set in some key and clear in another
-  uint64_t pad1[4];
+  uint64_t radix_nmin;  // signed radix keys: max over blocks of ~enc(x) (zero between calls)
+  uint64_t radix_base;  // signed radix keys: (U)min(x), the key of x being (U)x - base
+  uint64_t pad1[2];
   uint8_t value[64];  // reduce result slot (one element, written by the last block)
   uint32_t radix_skip;  // radix sorts: bit d set when every key shares digit d
   uint64_t radix_or[16];   // segmented radix: OR of the encoded keys, 16 slots (block % 16)
