@@ -604,7 +604,17 @@ size_t seg_sort_scratch_bytes(int dtype, const SegSpec& sp) {
 }
 
 int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp, void* ext_scratch,
-                 size_t ext_bytes) {
+                 size_t ext_bytes, hipStream_t pass_stream, hipEvent_t prep_done) {
+  // with a pass stream: whatever this call queued on c.stream() precedes later work there
+  auto join = [&](int rc) {
+    if (!pass_stream || !prep_done) return rc;
+    if (hipEventRecord(prep_done, c.stream()) != hipSuccess ||
+        hipStreamWaitEvent(pass_stream, prep_done, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      if (rc == PX_OK) rc = dev_err(hipErrorUnknown, "seg_sort: stream join");
+    }
+    return rc;
+  };
   if (!valid_dtype(dtype) || !data || !valid_spec(sp)) return PX_ERR_ARG;
   if (sp.nseg == 0) return PX_OK;
   ++g_stats.seg;
@@ -666,9 +676,9 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp, voi
   const uint64_t* bd = reinterpret_cast<const uint64_t*>(d);
   const uint32_t* tab_d = reinterpret_cast<const uint32_t*>(d + b_b);
   int rc = dev_err(pxk::seg_sort_run(p, data, bd, tab_d, c.stream()), "seg_sort");
-  if (rc != PX_OK || nl == 0) return rc;
+  if (rc != PX_OK || nl == 0) return join(rc);
   for (size_t i = 0; i < pxk::dtype_size(dtype); ++i)
-    if (!c.note_status_launch()) return PX_ERR_DEVICE;
+    if (!c.note_status_launch()) return join(PX_ERR_DEVICE);
   const pxk::Scratch sc = c.scratch();
   unsigned char* base = static_cast<unsigned char*>(ext_scratch ? ext_scratch : sc.partials);
   pxk::RadixSegs g;
@@ -683,7 +693,10 @@ int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp, voi
   g.tmp_b = base;
   g.tmp2 = base + buf_bytes;
   g.seg_cnt = reinterpret_cast<uint32_t*>(base + 2 * buf_bytes);
-  return dev_err(pxk::seg_radix_sort(dtype, data, g, sc, c.stream()), "seg_radix_sort");
+  rc = dev_err(pxk::seg_radix_sort(dtype, data, g, sc, c.stream(), pass_stream, prep_done), "seg_radix_sort");
+  // seg_radix_sort joins the streams itself once its counting kernels are queued; a
+  // failure before that still needs the join
+  return rc == PX_OK ? rc : join(rc);
 }
 
 }  // namespace parallax

@@ -143,8 +143,13 @@ int run_seg_scan(StreamContext& c, int dtype, const void* in, void* out, const S
 // seg_sort_scratch_bytes() of scratch: from ext_scratch (device memory of the caller,
 // e.g. the server's pool) when given, else from the context.
 size_t seg_sort_scratch_bytes(int dtype, const SegSpec& sp);
+// With `pass_stream` and `prep_done`, the digit passes (persistent look-back kernels)
+// run on pass_stream and everything else on c.stream(); every launch on c.stream() is
+// ordered before later work on pass_stream (prep_done recorded there and waited for),
+// whether or not any digit pass runs.
 int run_seg_sort(StreamContext& c, int dtype, void* data, const SegSpec& sp,
-                 void* ext_scratch = nullptr, size_t ext_bytes = 0);
+                 void* ext_scratch = nullptr, size_t ext_bytes = 0, hipStream_t pass_stream = nullptr,
+                 hipEvent_t prep_done = nullptr);
 int run_seg_compact(StreamContext& c, const px_desc& flags, const void* in, void* out,
                     const SegSpec& sp, uint64_t* counts_dev, pxk::TileCfg tile = {0, 0});
 // Default tile of a batched call: the largest candidate no longer than the mean

@@ -935,14 +935,30 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
 template <class T>
 __global__ __launch_bounds__(256) void k_segradix_min(const T* data_all, RadixSegs g, Ctl* ctl) {
   using U = typename RadixKey<T>::U;
+  constexpr int VEC = 16 / sizeof(T);
+  using V = Vec<T, VEC>;
   uint64_t nm = 0;
+  auto add = [&](T x) {
+    const uint64_t c = static_cast<U>(~RadixKey<T>::enc(x));
+    nm = c > nm ? c : nm;
+  };
   for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
     uint32_t sg, tb, te;
     seg_tile<T>(g, t, sg, tb, te);
     const T* data = data_all + g.seg_lo[sg];
-    for (uint32_t p = tb + threadIdx.x; p < te; p += 256) {
-      const uint64_t c = static_cast<U>(~RadixKey<T>::enc(__builtin_nontemporal_load(data + p)));
-      nm = c > nm ? c : nm;
+    // 16-byte loads for the tile's aligned body, single keys for its head and tail
+    const uint32_t mis = static_cast<uint32_t>(((reinterpret_cast<uintptr_t>(data + tb)) & 15u) / sizeof(T));
+    uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
+    if (head > te - tb) head = te - tb;
+    const uint32_t nvec = (te - tb - head) / VEC;
+    const uint32_t body_end = tb + head + nvec * VEC;
+    if (threadIdx.x < head) add(data[tb + threadIdx.x]);
+    if (threadIdx.x < te - body_end) add(data[body_end + threadIdx.x]);
+    const V* vp = reinterpret_cast<const V*>(data + tb + head);
+    for (uint32_t v = threadIdx.x; v < nvec; v += 256) {
+      const V x = ld_stream(vp + v);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) add(x.v[j]);
     }
   }
   radix_min_publish<T>(nm, ctl);
@@ -950,7 +966,7 @@ __global__ __launch_bounds__(256) void k_segradix_min(const T* data_all, RadixSe
 
 template <class T>
 hipError_t seg_radix_sort_t(void* data_v, const RadixSegs& g, const Scratch& s,
-                            hipStream_t stream) {
+                            hipStream_t stream, hipStream_t pass_stream, hipEvent_t prep_done) {
   if (g.ntiles == 0) return hipSuccess;
   T* data = static_cast<T*>(data_v);
   T* tmp = static_cast<T*>(g.tmp_b);
@@ -978,6 +994,11 @@ hipError_t seg_radix_sort_t(void* data_v, const RadixSegs& g, const Scratch& s,
   }
   k_segradix_hist<T><<<hgrid, 256, 0, stream>>>(data, g, s.ctl);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (pass_stream && prep_done) {
+    if ((e = hipEventRecord(prep_done, stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(pass_stream, prep_done, 0)) != hipSuccess) return e;
+    stream = pass_stream;
+  }
   const bool large = g.tile == static_cast<uint32_t>(kSegBlockLarge * kKptSeg<T>);
   const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(g.ntiles, large ? cus : 2 * cus));
   for (int d = 0; d < static_cast<int>(sizeof(T)); ++d) {
@@ -1044,8 +1065,8 @@ uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg) {
 }
 
 hipError_t seg_radix_sort(int dtype, void* data, const RadixSegs& seg, const Scratch& s,
-                          hipStream_t stream) {
-  PXK_DISPATCH(dtype, return seg_radix_sort_t<T>(data, seg, s, stream));
+                          hipStream_t stream, hipStream_t pass_stream, hipEvent_t prep_done) {
+  PXK_DISPATCH(dtype, return seg_radix_sort_t<T>(data, seg, s, stream, pass_stream, prep_done));
   return hipErrorInvalidValue;
 }
 

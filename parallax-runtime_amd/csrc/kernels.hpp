@@ -227,8 +227,12 @@ uint64_t seg_radix_min_len(int dtype);  // segments up to this length stay on th
 // ones (8K), i.e. unless many segments fill a small part of a large tile.
 uint32_t seg_radix_tile(int dtype, const uint64_t* lo, const uint64_t* hi, uint32_t n);
 uint64_t seg_radix_cnt_bytes(int dtype, uint32_t nseg);
+// The counting kernels run on `stream`; the digit passes (persistent look-back
+// kernels) on `pass_stream` when given, after `prep_done` is recorded on `stream` and
+// waited for there, else on `stream`.
 hipError_t seg_radix_sort(int dtype, void* data, const RadixSegs& seg, const Scratch& s,
-                          hipStream_t stream);
+                          hipStream_t stream, hipStream_t pass_stream = nullptr,
+                          hipEvent_t prep_done = nullptr);
 // Segmented sort of [seg_off[s], seg_off[s+1]) for every segment: a host plan (block
 // tables for each launch kind) uploaded by the caller, then a fixed launch sequence.
 struct SegSortPlan {

@@ -20,9 +20,11 @@
 // persistent look-back kernels sized to the whole device, and two of them on different
 // streams split the CUs: each ran 15x slower than alone (served pass, 8 slots on 8
 // streams: 7.8 ms; on one stream, one batch at a time: 5.9 ms; on one stream with
-// batches queued: 5.2 ms).  So they all take lane 0, and reduce batches (a static grid
-// with no waits between blocks, which shares CUs safely) take the last lane, where
-// they overlap the sorts' compute-bound passes (4.76 -> 4.65 ms, p50 4.0 -> 2.9 ms).
+// batches queued: 5.2 ms).  So they all take lane 0, and what has no waits between
+// blocks (a static grid, which shares CUs safely) takes the last lane: reduce batches,
+// and a sort batch's table upload, short-segment sort and counting kernels, after which
+// an event sends its digit passes to lane 0.  Those overlap the look-back kernels of the
+// batches ahead on lane 0 (reduces: 4.76 -> 4.65 ms, p50 4.0 -> 2.9 ms).
 //
 // When nothing runs and the queue head waits for memory only the client can free
 // (outputs it has not released), the server is stalled: pxs_wait on a request that is
@@ -76,7 +78,9 @@ struct Req {
 
 struct Slot {
   std::unique_ptr<StreamContext> ctx;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr;       // the batch is complete
+  hipEvent_t ev_prep = nullptr;  // a sort's counting kernels are done (its passes wait)
+  hipStream_t done_stream = nullptr;  // where `ev` is recorded (the lane of the last launch)
   bool busy = false;
   int rc = PX_EOK;               // status the batch completes with
   std::vector<int64_t> ids;
@@ -242,7 +246,12 @@ int pxs_server::launch(Slot& s, const std::vector<int64_t>& ids) {
   switch (r0.r.op) {
     case PX_OP_REDUCE: rc = parallax::run_seg_reduce(*s.ctx, r0.r.dtype, b, sp, b, kServedReduceTile); break;
     case PX_OP_SCAN: rc = parallax::run_seg_scan(*s.ctx, r0.r.dtype, b, b, sp, pxk::tune().scan[r0.r.dtype]); break;
-    case PX_OP_SORT: rc = parallax::run_seg_sort(*s.ctx, r0.r.dtype, b, sp, s.scratch, scratch_bytes); break;
+    case PX_OP_SORT:
+      // with two lanes the slot is on the last: its counting kernels run there and its
+      // digit passes (look-back kernels) on lane 0
+      rc = parallax::run_seg_sort(*s.ctx, r0.r.dtype, b, sp, s.scratch, scratch_bytes,
+                                  lanes.size() > 1 ? lanes[0] : nullptr, lanes.size() > 1 ? s.ev_prep : nullptr);
+      break;
     case PX_OP_FLAGS:
       rc = parallax::run_seg_compact(*s.ctx, r0.r.flags, b, b, sp, static_cast<uint64_t*>(s.counts),
                                      pxk::tune().compact[r0.r.flags.dtype]);
@@ -254,10 +263,13 @@ int pxs_server::launch(Slot& s, const std::vector<int64_t>& ids) {
     default: rc = PX_EARG;
   }
   // Kernels queued before a failure may still run on the request buffers: the batch
-  // completes (with rc) when the slot's event fires, never before.
-  if (hipEventRecord(s.ev, s.ctx->stream()) != hipSuccess) {
+  // completes (with rc) when the slot's event fires, never before.  A sort on two lanes
+  // ends on lane 0 (everything it queued on its own lane is ordered before that).
+  s.done_stream = (r0.r.op == PX_OP_SORT && lanes.size() > 1) ? lanes[0] : s.ctx->stream();
+  if (hipEventRecord(s.ev, s.done_stream) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamSynchronize(s.ctx->stream());
+    (void)hipStreamSynchronize(s.done_stream);
     return fail_now(rc != PX_EOK ? rc : PX_EDEVICE);
   }
   s.ids = ids;
@@ -308,19 +320,26 @@ void pxs_server::loop() {
     }
     peak = std::max(peak, pool.used());
     // 3. dispatch to free slots (slot i on lane i % lanes; with two or more lanes, reduce
-    // batches take the last lane and the others lane 0)
+    // and sort batches take the last lane -- a sort's digit passes then run on lane 0 --
+    // and the others lane 0)
     bool dispatch_waits = false;
     const size_t nl = lanes.size();
     for (size_t si = 0; si < slots.size(); ++si) {
       Slot& s = slots[si];
       if (s.busy || ready.empty()) continue;
       const size_t lane = si % nl;
-      auto lane_of = [&](const Req& q) { return (nl > 1 && q.r.op == PX_OP_REDUCE) ? nl - 1 : size_t(0); };
+      auto lane_of = [&](const Req& q) {
+        return (nl > 1 && (q.r.op == PX_OP_REDUCE || q.r.op == PX_OP_SORT)) ? nl - 1 : size_t(0);
+      };
+      // the oldest request of this lane; on the last lane of several, the oldest sort
+      // first (its digit passes wait on lane 0 for its counting kernels)
       int64_t head_id = -1;
       for (int64_t id : ready)
-        if (lane_of(reqs[id]) == lane) {
+        if (lane_of(reqs[id]) == lane && (head_id < 0 || (nl > 1 && lane == nl - 1 &&
+                                                           reqs[id].r.op == PX_OP_SORT &&
+                                                           reqs[head_id].r.op != PX_OP_SORT))) {
           head_id = id;
-          break;
+          if (nl == 1 || lane != nl - 1 || reqs[id].r.op == PX_OP_SORT) break;
         }
       if (head_id < 0) continue;
       const Req& head = reqs[head_id];
@@ -437,7 +456,8 @@ pxs_server_t pxs_create(const pxs_config* cfg) {
   for (size_t i = 0; i < s->slots.size(); ++i) {
     Slot& sl = s->slots[i];
     sl.ctx = std::make_unique<StreamContext>(be, s->lanes[i % s->lanes.size()]);
-    if (!sl.ctx->ok() || hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+    if (!sl.ctx->ok() || hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.ev_prep, hipEventDisableTiming) != hipSuccess) {
       parallax::set_last_error(PX_ENOMEM, "server slot creation failed");
       pxs_destroy(s);
       return nullptr;
@@ -457,7 +477,9 @@ void pxs_destroy(pxs_server_t s) {
   if (s->worker.joinable()) s->worker.join();
   for (auto& sl : s->slots) {
     if (sl.ctx) (void)hipStreamSynchronize(sl.ctx->stream());
+    if (sl.done_stream) (void)hipStreamSynchronize(sl.done_stream);
     if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.ev_prep) (void)hipEventDestroy(sl.ev_prep);
     if (sl.pin_counts) (void)hipHostFree(sl.pin_counts);
     sl.ctx.reset();
   }
