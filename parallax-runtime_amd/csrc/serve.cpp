@@ -324,7 +324,12 @@ void pxs_server::loop() {
     // and the others lane 0)
     bool dispatch_waits = false;
     const size_t nl = lanes.size();
-    for (size_t si = 0; si < slots.size(); ++si) {
+    // lane 0's slots first: its own batches then queue ahead of the digit passes that
+    // sorts dispatched on the last lane add to it (those wait for their counting kernels)
+    for (size_t q = 0; q < slots.size(); ++q) {
+      const size_t per = (slots.size() + nl - 1) / nl;
+      const size_t si = (q % per) * nl + q / per;  // lane-major order over slot indices
+      if (si >= slots.size()) continue;
       Slot& s = slots[si];
       if (s.busy || ready.empty()) continue;
       const size_t lane = si % nl;
