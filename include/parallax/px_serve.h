@@ -43,8 +43,8 @@ typedef struct pxs_config {
   size_t hbm_budget;    /* bytes of the device pool (inputs + outputs); default 64 GiB */
   int max_batch;        /* most requests fused into one launch; default 4096 */
   double max_wait_us;   /* how long a lone request may wait for batch-mates; default 0 */
-  int lanes;            /* streams the slots share (1..slots); default 2: reduce batches
-                           on the last one, every other batch on the first */
+  int lanes;            /* streams the slots share, 1 or 2 (more count as 2); default 2:
+                           reduces and sorts' counting on the second, the rest on the first */
 } pxs_config;
 
 typedef struct pxs_request {

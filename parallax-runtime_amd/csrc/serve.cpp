@@ -435,7 +435,7 @@ pxs_server_t pxs_create(const pxs_config* cfg) {
   if (cfg) s->cfg = *cfg;
   if (s->cfg.slots < 1) s->cfg.slots = 1;
   if (s->cfg.max_batch < 1) s->cfg.max_batch = 1;
-  s->cfg.lanes = std::min(std::max(s->cfg.lanes, 1), s->cfg.slots);
+  s->cfg.lanes = std::min(std::max(s->cfg.lanes, 1), std::min(s->cfg.slots, 2));  // a third lane gets no work
   s->be = be;
   if (!s->pool.initialize(be, s->cfg.hbm_budget, false)) {
     parallax::set_last_error(PX_ENOMEM, "server pool allocation failed");
