@@ -413,12 +413,14 @@ def test_graph_plan_replays(gpu_ctx):
     other.close()
 
 
-def make_server(budget, slots=4, max_batch=4096):
+def make_server(budget, slots=4, max_batch=4096, lanes=None):
     cfg = PxsConfig()
     lib.pxs_config_default(ctypes.byref(cfg))
     cfg.hbm_budget = budget
     cfg.slots = slots
     cfg.max_batch = max_batch
+    if lanes is not None:
+        cfg.lanes = lanes
     s = lib.pxs_create(ctypes.byref(cfg))
     assert s, "pxs_create failed"
     return s
@@ -495,8 +497,11 @@ def submit(server, op, dt, a, p):
     return rid, fl
 
 
-def test_served_mix_matches_oracle_and_alone():
-    server = make_server(1 << 30)
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_served_mix_matches_oracle_and_alone(lanes):
+    """One stream, or two (reduces and the sorts' counting kernels on the second, the
+    sorts' digit passes joining the first by an event)."""
+    server = make_server(1 << 30, lanes=lanes)
     reqs = mixed_requests(4, 120)
     ptrs = [put(server, a) for _, _, a in reqs]
     # all at once (mixed, batched, overlapping)
