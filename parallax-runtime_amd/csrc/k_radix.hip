@@ -504,17 +504,29 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   const UK base = kSignedKey<T> ? static_cast<UK>(ld_agent(&ctl->radix_base)) : UK(0);
 
   if (threadIdx.x == 0) s_epoch = ld_agent32(&ctl->epoch);
+  __syncthreads();
+  const uint32_t epoch = s_epoch;
 
   // Persistent loop over tickets; a tile waits only on lower tickets, all of which are
-  // held by running blocks, so every wait ends.
+  // held by running blocks, so every wait ends.  Thread 0 takes the next ticket before
+  // the tile's write-out, so the wait for it at the top of the loop does not include
+  // the write-out's stores (a wait on a return waits for every memory operation issued
+  // before it).  The address is taken as a per-lane value, so the fetch stays one lane's
+  // atomic (a uniform one is folded into a wave-wide form that waits for its return at
+  // once).
+  auto take_ticket = [&]() {
+    uint32_t zero = 0;
+    asm volatile("" : "+v"(zero));
+    return __hip_atomic_fetch_add(&ctl->ticket + zero, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  if (threadIdx.x == 0) s_tile = take_ticket();
+  __syncthreads();
   while (true) {
-    if (threadIdx.x == 0)
-      s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the previous tile's last reads of s_wh came before the end-of-tile barrier, and a
+    // wave's digit counts are its own until the ranking barrier
 #pragma unroll
     for (int q = 0; q < kBins / kWave; ++q) s_wh[wid][q * kWave + lane] = 0u;
-    __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t epoch = s_epoch;
     if (tile >= ntiles) break;
     // Positions in 32-bit arithmetic (n < 2^32; hipcc lowered a 64-bit form of the
     // tile-length select -- a vector compare, then a scalar select on SCC -- to a select
@@ -550,8 +562,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // ---- loads (a wave's key j of all lanes is 64 consecutive keys)
     T key[KPT];
     const T* lp = tin + first;
-#pragma unroll
-    for (int j = 0; j < KPT; ++j) key[j] = j < nv ? __builtin_nontemporal_load(lp + j * kWave) : radix_pad<T>(base);
+    const bool full = tend - tbase == static_cast<uint32_t>(TILE);
 
     // ---- rank among equal digits of the wave, in input order (two 16-bit ranks per
     // register: a tile holds fewer than 2^16 keys)
@@ -583,10 +594,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         if (below == 0) s_wh[wid][d] = c + static_cast<uint32_t>(__popcll(peers));
       }
     };
-    if (SEG && vary != 0xffu && tend - tbase == static_cast<uint32_t>(TILE))
-      rank_keys(std::true_type{});
-    else
+    if (full) {
+      // no conditions between the loads and the ranking: key j is ranked as soon as it
+      // has arrived, while the later ones are still in flight
+#pragma unroll
+      for (int j = 0; j < KPT; ++j) key[j] = __builtin_nontemporal_load(lp + j * kWave);
+      if (SEG && vary != 0xffu)
+        rank_keys(std::true_type{});
+      else
+        rank_keys(std::false_type{});
+    } else {
+#pragma unroll
+      for (int j = 0; j < KPT; ++j) key[j] = j < nv ? __builtin_nontemporal_load(lp + j * kWave) : radix_pad<T>(base);
       rank_keys(std::false_type{});
+    }
     __syncthreads();
 
     // ---- per digit: wave offsets, tile count (published), block-wide exclusive scan
@@ -677,21 +698,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
     __syncthreads();
 
-    // ---- write out: consecutive tile-local indices of one digit are consecutive outputs
-    const uint32_t tn = tend - tbase;
-    if (tn == TILE) {
+    uint32_t next_tile = 0;
+    if (threadIdx.x == 0) next_tile = take_ticket();
+
+    // ---- write out: consecutive tile-local indices of one digit are consecutive outputs.
+    // Every thread issues exactly KPT stores, so the wait for the ticket below counts
+    // them out: in a partial tile, indices past its end store the tile's last key to its
+    // place once more (the same value to the same address).
+    const uint32_t tlast = tend - tbase - 1u;
 #pragma unroll
-      for (int i0 = 0; i0 < KPT; ++i0) {
-        const uint32_t i = static_cast<uint32_t>(i0) * BLOCK + threadIdx.x;
-        const T k = s_keys[i];
-        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift, base)] + i)] = k;
-      }
-    } else {
-      for (uint32_t i = threadIdx.x; i < tn; i += BLOCK) {
-        const T k = s_keys[i];
-        tout[static_cast<uint32_t>(s_goff[digit_of(k, shift, base)] + i)] = k;
-      }
+    for (int i0 = 0; i0 < KPT; ++i0) {
+      const uint32_t i = min(static_cast<uint32_t>(i0) * BLOCK + threadIdx.x, tlast);
+      const T k = s_keys[i];
+      tout[static_cast<uint32_t>(s_goff[digit_of(k, shift, base)] + i)] = k;
     }
+    // every thread has read this tile's s_tile (before the ranking barrier)
+    if (threadIdx.x == 0) s_tile = next_tile;
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -699,7 +721,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (d == gridDim.x - 1) {  // last block: reset for the next launch
       st_agent32(&ctl->ticket, 0u);
       st_agent32(&ctl->done, 0u);
-      st_agent32(&ctl->epoch, next_epoch(s_epoch));
+      st_agent32(&ctl->epoch, next_epoch(epoch));
     }
   }
 }
