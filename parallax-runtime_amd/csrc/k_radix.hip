@@ -576,18 +576,28 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         // Masked: a bit every key of the batch shares (all 64 lanes hold real keys)
         // agrees in every lane and is skipped (a uniform branch).
         uint32_t diff_lo = 0, diff_hi = 0;
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          if (decltype(masked)::value && !((vary >> b) & 1u)) continue;
+        // diff |= m ^ own in one three-input op per half (truth table 0xF6 for
+        // S0 | (S1 ^ S2) with S0, S1, S2 = 0xF0, 0xCC, 0xAA); 0x09 = ~(S0 | (S1 ^ S2))
+        // gives the complement, the peers, at once.
+        auto add_bit = [&](int b, auto table) {
           uint32_t own = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int32_t>(d), b, 1));
           asm volatile("" : "+v"(own));  // ballot from the extracted bit: one compare
           const uint64_t m = __ballot(own != 0u);
-          // diff |= m ^ own in one three-input op per half (truth table 0xF6 for
-          // S0 | (S1 ^ S2) with S0, S1, S2 = 0xF0, 0xCC, 0xAA).
-          diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, static_cast<uint32_t>(m), own, 0xF6);
-          diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, static_cast<uint32_t>(m >> 32), own, 0xF6);
+          diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, static_cast<uint32_t>(m), own, decltype(table)::value);
+          diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, static_cast<uint32_t>(m >> 32), own, decltype(table)::value);
+        };
+        uint64_t peers;
+        if constexpr (decltype(masked)::value) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b)
+            if ((vary >> b) & 1u) add_bit(b, std::integral_constant<int, 0xF6>{});
+          peers = ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
+        } else {
+#pragma unroll
+          for (int b = 0; b < 7; ++b) add_bit(b, std::integral_constant<int, 0xF6>{});
+          add_bit(7, std::integral_constant<int, 0x09>{});
+          peers = (static_cast<uint64_t>(diff_hi) << 32) | diff_lo;
         }
-        const uint64_t peers = ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
         const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below));
         const uint32_t c = s_wh[wid][d];
         rank2[j / 2] |= (c + below) << (16 * (j & 1));
