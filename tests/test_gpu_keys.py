@@ -7,6 +7,8 @@
 * Signed integer radix sorts key on x - min(x): narrow ranges of either sign (whose
   upper digits are then shared and skipped), ranges at the ends of the type, the full
   range and all-equal arrays, plain (radix path) and batched (segmented radix path).
+* Digits whose keys share some bits, plain radix path: every byte (random & mask) |
+  fixed, and uniform [0, 1) doubles (whose top digit takes two values).
 """
 import ctypes
 
@@ -16,7 +18,7 @@ import pytest
 import pxo
 from parallax_amd import lib, check, make_desc, consts as C
 from parallax_amd.device import DeviceBuffer
-from parallax_amd.dtypes import NP_DTYPES, PX_I32, PX_I64, PX_U32, PX_U64
+from parallax_amd.dtypes import NP_DTYPES, PX_F64, PX_I32, PX_I64, PX_U32, PX_U64
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +100,33 @@ def test_signed_segmented_radix_key_base(gpu_ctx, dt):
               "seg_sort")
         gpu_ctx.sync()
         assert b.download(a.dtype, a.size).tobytes() == pxo.seg_sort(a, off).tobytes()
+
+
+@pytest.mark.parametrize("dt", [PX_U32, PX_U64, PX_I32])
+@pytest.mark.parametrize("mask_set", [(0x0F, 0x30), (0x81, 0x00), (0x01, 0xFE), (0x7F, 0x80)])
+def test_radix_partially_varying_digits(gpu_ctx, dt, mask_set):
+    """Digits whose keys share some bits (a digit all keys share is skipped): each byte
+    of every key is (random & mask) | fixed; full tiles and a partial last tile."""
+    m, f = mask_set
+    np_t = NP_DTYPES[dt]
+    size = np.dtype(np_t).itemsize
+    n = (1 << 20) + 5  # the radix path for both widths, a partial last tile
+    rng = np.random.default_rng(41 + dt + m)
+    b = (rng.integers(0, 256, (n, size), dtype=np.uint8) & m) | f
+    a = b.view(np_t).reshape(n).copy()
+    d = DeviceBuffer.from_array(a)
+    check(lib.pxd_sort(gpu_ctx.h, dt, d.ptr, n), "sort")
+    gpu_ctx.sync()
+    assert d.download(a.dtype, n).tobytes() == pxo.sort(a).tobytes()
+
+
+def test_radix_f64_unit_interval_top_digit(gpu_ctx):
+    """Uniform [0, 1) doubles (the top digit takes two values), with 0, a tiny value and
+    the largest double below 1."""
+    n = (1 << 21) + 77
+    a = np.random.default_rng(43).random(n)
+    a[:3] = [0.0, 2.0 ** -40, 1.0 - 2.0 ** -53]
+    d = DeviceBuffer.from_array(a)
+    check(lib.pxd_sort(gpu_ctx.h, PX_F64, d.ptr, n), "sort")
+    gpu_ctx.sync()
+    assert d.download(a.dtype, n).tobytes() == pxo.sort(a).tobytes()
