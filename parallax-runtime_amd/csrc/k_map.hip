@@ -236,9 +236,12 @@ __device__ __forceinline__ bool better(const Best<T>& a, const Best<T>& b, bool 
   return want_last ? (a.i > b.i) : (a.i < b.i);
 }
 
-// A thread visits its elements in increasing index order, so one comparison per element
-// keeps its winner (strict for the first index among ties, non-strict for the last);
-// winners of different threads combine by better().
+// A thread keeps one winner per vector slot j (its elements at vector position j, in
+// increasing index order), so one comparison per element keeps that slot's winner
+// (strict for the first index among ties, non-strict for the last) and the VEC chains
+// of comparisons are independent; a winner's index is its 32-bit vector ordinal within
+// the thread.  The slots' winners, the head and tail elements and the winners of other
+// threads combine by better().
 template <class T, bool MAX, bool LAST>
 __global__ __launch_bounds__(256) void k_argminmax(const T* in_al, int64_t lo, int64_t hi,
                                                    Best<T>* partials, Ctl* ctl, uint64_t* out) {
@@ -249,46 +252,63 @@ __global__ __launch_bounds__(256) void k_argminmax(const T* in_al, int64_t lo, i
   __shared__ Best<T> s_b[256];
   __shared__ int s_last;
   const uint64_t n = static_cast<uint64_t>(hi - lo);
-  Best<T> b{T{}, n};
-  bool have = false;
+  Best<T> b{T{}, n};  // index n: none yet
   auto take = [&](T x, int64_t pos) {
-    bool win;
+    const Best<T> c{x, static_cast<uint64_t>(pos - lo)};
+    if (better(c, b, want_max, want_last, n)) b = c;
+  };
+  auto wins = [](T x, T cur) {
     if constexpr (MAX)
-      win = LAST ? !(x < b.v) : (b.v < x);
+      return LAST ? !(x < cur) : (cur < x);
     else
-      win = LAST ? !(b.v < x) : (x < b.v);
-    if (!have || win) {
-      b.v = x;
-      b.i = static_cast<uint64_t>(pos - lo);
-      have = true;
-    }
+      return LAST ? !(cur < x) : (x < cur);
   };
   const int64_t va = (lo + VEC - 1) / VEC;  // interior vectors [va, vb)
   const int64_t vb = hi / VEC;
-  // the head (before the first whole vector) first and the tail last, so every thread
-  // still sees its elements in increasing index order
-  if (blockIdx.x == 0 && threadIdx.x < VEC) {
+  if (blockIdx.x == 0 && threadIdx.x < VEC) {  // the head (before the first whole vector)
     const int64_t pos = lo + threadIdx.x;
     if (pos < va * VEC && pos < hi) take(in_al[pos], pos);
   }
   const V* vin = reinterpret_cast<const V*>(in_al);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
-  int64_t v = va + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  for (; v + (UL - 1) * stride < vb; v += UL * stride) {
+  const int64_t v0 = va + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  int64_t v = v0;
+  T bv[VEC];
+  uint32_t bk[VEC];  // vector ordinal of slot j's winner; none when the thread has no vector
+  constexpr uint32_t kNone = 0xffffffffu;
+#pragma unroll
+  for (int j = 0; j < VEC; ++j) bk[j] = kNone;
+  uint32_t k = 0;
+  if (v < vb) {  // the thread's first vector seeds every slot
+    const V x = ld_stream(vin + v);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      bv[j] = x.v[j];
+      bk[j] = 0;
+    }
+    v += stride;
+    k = 1;
+  }
+  auto step = [&](const V& x, uint32_t kx) {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const bool w = wins(x.v[j], bv[j]);
+      bv[j] = w ? x.v[j] : bv[j];
+      bk[j] = w ? kx : bk[j];
+    }
+  };
+  for (; v + (UL - 1) * stride < vb; v += UL * stride, k += UL) {
     V x[UL];
 #pragma unroll
     for (int u = 0; u < UL; ++u) x[u] = ld_stream(vin + v + u * stride);
 #pragma unroll
-    for (int u = 0; u < UL; ++u)
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) take(x[u].v[j], (v + u * stride) * VEC + j);
+    for (int u = 0; u < UL; ++u) step(x[u], k + static_cast<uint32_t>(u));
   }
-  for (; v < vb; v += stride) {
-    const V x = ld_stream(vin + v);
+  for (; v < vb; v += stride, ++k) step(ld_stream(vin + v), k);
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) take(x.v[j], v * VEC + j);
-  }
-  if (blockIdx.x == 0 && threadIdx.x < VEC) {
+  for (int j = 0; j < VEC; ++j)
+    if (bk[j] != kNone) take(bv[j], (v0 + static_cast<int64_t>(bk[j]) * stride) * VEC + j);
+  if (blockIdx.x == 0 && threadIdx.x < VEC) {  // the tail (after the last whole vector)
     const int64_t pos = (va > vb ? va : vb) * VEC + threadIdx.x;
     if (pos < hi) take(in_al[pos], pos);
   }
@@ -444,10 +464,11 @@ hipError_t argminmax(int dtype, const void* in, uint64_t n, bool want_max, bool 
     return hipGetLastError();
   }
   PXK_DISPATCH(dtype, {
-    // four 256-thread blocks per CU at most (measured: 2 and 8 per CU slower), 8 vectors
-    // per thread per round
+    // one 256-thread block per CU at most, 8 vectors per thread per round (2^28 f32 /
+    // i32: 1 / 2 / 4 blocks per CU 0.167 / 0.210 / 0.278 ms; with one comparison chain
+    // per thread instead of one per vector slot, 4 per CU was best at 0.260 ms)
     const uint64_t per_block = 256ull * 8 * (16 / sizeof(T));
-    const uint64_t cap = s.max_blocks / 2 ? s.max_blocks / 2 : 1;
+    const uint64_t cap = s.max_blocks / 8 ? s.max_blocks / 8 : 1;
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>((n + per_block - 1) / per_block, 1), cap));
     if (grid * sizeof(Best<T>) > s.partial_bytes) return hipErrorInvalidValue;
     int64_t lo;
