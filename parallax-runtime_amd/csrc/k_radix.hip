@@ -450,11 +450,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     } else {
       if ((skip >> dig) & 1u) return;  // every key shares this digit
       const int i = __builtin_popcount(runs & ((1u << dig) - 1u));  // this pass's run index
-      T* const buf[3] = {a, b, static_cast<T*>(g.tmp2)};
+      // buffer 0 = a, 1 = b, 2 = tmp2, picked by selects (an indexed array of the
+      // pointers would lose that they are global memory: flat loads and stores)
+      auto buf = [&](int j) { return j == 0 ? a : (j == 1 ? b : static_cast<T*>(g.tmp2)); };
       auto dst = [&](int j) { return j == k - 1 ? 0 : ((k & 1) ? 1 + (j & 1) : 1 - (j & 1)); };
       const int bi = i == 0 ? 0 : dst(i - 1), bo = dst(i);
-      in = buf[bi];
-      out = buf[bo];
+      in = buf(bi);
+      out = buf(bo);
       in_a = bi == 0;
       out_a = bo == 0;
     }

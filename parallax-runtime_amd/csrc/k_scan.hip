@@ -54,7 +54,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
   static_assert(NW <= 16, "the wave totals are scanned in one 16-lane DPP row");
   using V = Vec<T, VEC>;
   const V* vin = reinterpret_cast<const V*>(in_al);
-  V* vout = reinterpret_cast<V*>(reinterpret_cast<uintptr_t>(out) & ~uintptr_t(15));
+  // the 16-byte-aligned address at or below `out`, by pointer arithmetic (through an
+  // integer the compiler loses that it is global memory and stores through flat
+  // instructions, which the LDS wait counter also counts)
+  V* vout = reinterpret_cast<V*>(out - out_shift);
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_epoch;
