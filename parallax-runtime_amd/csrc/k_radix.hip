@@ -493,7 +493,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   __shared__ uint32_t s_goff[kBins];    // output index of tile-local index 0, per digit
   __shared__ uint32_t s_wsum[kBins / kWave];
   __shared__ uint32_t s_bsum[kBins / kWave];  // segment bin-start scan (SEG)
-  __shared__ T s_keys[TILE];
+  __shared__ __attribute__((aligned(16))) T s_keys[TILE];
 
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
@@ -528,6 +528,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // wave's digit counts are its own until the ranking barrier
 #pragma unroll
     for (int q = 0; q < kBins / kWave; ++q) s_wh[wid][q * kWave + lane] = 0u;
+    // the wave's peer-mask table (256 64-bit lane masks, zero between keys), in the tile
+    // buffer, which holds nothing until the reorder
+    uint64_t* const mt = reinterpret_cast<uint64_t*>(s_keys) + wid * kBins;
+#pragma unroll
+    for (int q = 0; q < kBins / kWave; ++q) mt[q * kWave + lane] = 0ull;
     const uint32_t tile = s_tile;
     if (tile >= ntiles) break;
     // Positions in 32-bit arithmetic (n < 2^32; hipcc lowered a 64-bit form of the
@@ -595,10 +600,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             if ((vary >> b) & 1u) add_bit(b, std::integral_constant<int, 0xF6>{});
           peers = ~((static_cast<uint64_t>(diff_hi) << 32) | diff_lo);
         } else {
+          // Some keys find their peers through LDS instead: every lane ORs its bit into
+          // its digit's mask, reads the mask back and clears it (a wave's LDS operations
+          // run in order, so the read sees all 64 ORs and the next key a cleared word):
+          // three LDS operations and no ballots, so ranking alternates between the VALU
+          // and LDS units (every second 4-byte key, every third 8-byte key: 2^26 u32
+          // 0.94 -> 0.88 ms, 2^25 f64 1.29 -> 1.28 ms; all keys through LDS is no faster
+          // than none)
+          if (sizeof(T) == 4 ? (j & 1) != 0 : j % 3 == 2) {
+            __hip_atomic_fetch_or(&mt[d], 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            peers = __hip_atomic_load(&mt[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&mt[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
 #pragma unroll
-          for (int b = 0; b < 7; ++b) add_bit(b, std::integral_constant<int, 0xF6>{});
-          add_bit(7, std::integral_constant<int, 0x09>{});
-          peers = (static_cast<uint64_t>(diff_hi) << 32) | diff_lo;
+            for (int b = 0; b < 7; ++b) add_bit(b, std::integral_constant<int, 0xF6>{});
+            add_bit(7, std::integral_constant<int, 0x09>{});
+            peers = (static_cast<uint64_t>(diff_hi) << 32) | diff_lo;
+          }
         }
         const uint32_t below = static_cast<uint32_t>(__popcll(peers & lanes_below));
         const uint32_t c = s_wh[wid][d];
