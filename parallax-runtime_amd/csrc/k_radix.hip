@@ -127,12 +127,14 @@ __device__ __forceinline__ typename RadixKey<T>::U read_key_base(const Ctl* ctl)
 }
 
 // Keys per lane of a whole-array digit pass: 16 for 8-byte keys (1024-thread tiles of 16K
-// keys, 128 KiB of LDS), 20 for 4-byte keys (the most without spills: 22 spills 6
-// VGPRs, 24 spills 17).  Measured: 8-byte keys at 8 per lane 1.85 ms, at 16 per lane
+// keys, 128 KiB of LDS; 18 would need 161 KiB), 22 for 4-byte keys (the most without
+// spills: 24 spills).  Measured: 8-byte keys at 8 per lane 1.85 ms, at 16 per lane
 // 1.51 ms (2^25 f64); 4-byte keys at 16 / 18 / 20 per lane 1.08 / 1.06 / 1.02 ms (2^26
-// u32: fewer tiles, fewer look-backs).  The segmented pass keeps 16 (at 20 it spills).
+// u32: fewer tiles, fewer look-backs), and with the LDS peer-mask ranking 20 / 22 per
+// lane 0.886 / 0.869 ms (2^26 u32), 0.228 / 0.219 ms (2^24 i32 in [-1000, 1000]).  The
+// segmented pass keeps 16 (at 20 it spills).
 template <class T>
-constexpr int kKpt = sizeof(T) == 4 ? 20 : 16;
+constexpr int kKpt = sizeof(T) == 4 ? 22 : 16;
 template <class T>
 constexpr int kKptSeg = 16;
 
