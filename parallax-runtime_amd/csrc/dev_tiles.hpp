@@ -41,9 +41,20 @@ struct TileInfo {
   int64_t out_delta;    // output element index = input element index + out_delta
 };
 
+// Entry i of a table that no kernel writes while it runs, at a wave-uniform index, as a
+// scalar load (constant address space).  A vector load would count in the same in-order
+// counter as the tile's stores, and a wait for it would also wait for every store issued
+// before it.
+template <class T>
+__device__ __forceinline__ T ld_table(const T* p, uint32_t i) {
+  using CP = const __attribute__((address_space(4))) T*;
+  return ((CP)(p))[__builtin_amdgcn_readfirstlane(i)];
+}
+
 template <int VEC, int TILE>
 __device__ __forceinline__ TileInfo tile_info(const SegGeom& g, uint32_t tile, int64_t shift) {
   TileInfo t;
+  tile = __builtin_amdgcn_readfirstlane(tile);
   if (g.seg_lo == nullptr) {
     t.seg = 0;
     t.first_tile = 0;
@@ -53,15 +64,15 @@ __device__ __forceinline__ TileInfo tile_info(const SegGeom& g, uint32_t tile, i
     t.w0 = static_cast<int64_t>(tile) * TILE;
     t.out_delta = 0;
   } else {
-    t.seg = g.tile_seg[tile];
-    t.first_tile = g.tile_off[t.seg];
-    t.last_tile = g.tile_off[t.seg + 1] - 1;
-    const int64_t b = static_cast<int64_t>(g.seg_lo[t.seg]);
+    t.seg = ld_table(g.tile_seg, tile);
+    t.first_tile = ld_table(g.tile_off, t.seg);
+    t.last_tile = ld_table(g.tile_off, t.seg + 1) - 1;
+    const int64_t b = static_cast<int64_t>(ld_table(g.seg_lo, t.seg));
     t.lo = shift + b;
-    t.hi = shift + static_cast<int64_t>(g.seg_hi[t.seg]);
+    t.hi = shift + static_cast<int64_t>(ld_table(g.seg_hi, t.seg));
     t.w0 = (t.lo & ~static_cast<int64_t>(VEC - 1)) +
            static_cast<int64_t>(tile - t.first_tile) * TILE;
-    t.out_delta = g.seg_out ? static_cast<int64_t>(g.seg_out[t.seg]) - b : 0;
+    t.out_delta = g.seg_out ? static_cast<int64_t>(ld_table(g.seg_out, t.seg)) - b : 0;
   }
   return t;
 }

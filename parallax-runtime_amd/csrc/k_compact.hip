@@ -102,7 +102,10 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
 
   const Pred<T> pred(pd);
   const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
+  // Wave index, tickets and tile geometry are wave-uniform and read as such (scalar
+  // registers, scalar branches): a branch the compiler takes for divergent runs both arms
+  // under lane masks, and its waits then cover both arms' loads.
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 
   // Positions inside the tile window are handled as 32-bit offsets rel in [0, TILE):
   // thread element e = k*VEC + j sits at rel = base_rel(k) + j; the segment covers
@@ -162,6 +165,9 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
         for (int j = 0; j < VEC; ++j)
           if (r0 + j >= u.lo_r && r0 + j < u.hi_r) u.valid |= 1u << (k * VEC + j);
       }
+      // a partial tile's loads complete here (vmcnt 0), so the full tiles' loads into
+      // the same registers need no wait of their own (the arms join before them)
+      __builtin_amdgcn_s_waitcnt(0x0F70);
     }
   };
 
@@ -286,11 +292,12 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
 
   // Persistent loop over tickets.  The block takes its next ticket while ranking the
   // current tile and issues that tile's loads once the current one is staged in LDS,
-  // before the look-back and the writes, so their latency overlaps.  A block holds at most two tickets and finishes the lower
-  // first; the lowest unfinished ticket waits only on finished ones, so every wait
-  // ends.  In place (out == in) stays safe: a tile's outputs land below its own end,
-  // and its look-back ends only after every earlier tile has loaded.
-  uint32_t tile = s_tile;
+  // before the look-back and the writes, so their latency overlaps.  A block holds at
+  // most two tickets and finishes the lower first; the lowest unfinished ticket waits
+  // only on finished ones, so every wait ends.  In place (out == in) stays safe: a
+  // tile's outputs land below its own end, and its look-back ends only after every
+  // earlier tile has loaded.
+  uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
   if (tile < g.ntiles) {
     TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
     Tile a;
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
       stage(a);
       // the tile's values now sit in LDS: their registers take the next tile's loads
       __builtin_amdgcn_sched_barrier(0);
-      const uint32_t next = s_next;
+      const uint32_t next = __builtin_amdgcn_readfirstlane(s_next);
       const bool more = next < g.ntiles;
       TileInfo tn = ti;
       Tile b;
@@ -338,9 +345,10 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
       }
       __syncthreads();
 
-      // ---- coalesced writes of the staged run(s)
       const uint64_t P = s_prefix;
-      const uint64_t seg_out = (g.seg_lo != nullptr) ? (g.seg_out ? g.seg_out[ti.seg] : g.seg_lo[ti.seg]) : 0;
+      const uint64_t seg_out =
+          (g.seg_lo != nullptr) ? (g.seg_out ? ld_table(g.seg_out, ti.seg) : ld_table(g.seg_lo, ti.seg)) : 0;
+      // ---- coalesced writes of the staged run(s)
       write(a, ti, seg_out, P);
       __syncthreads();
       if (!more) break;

@@ -66,7 +66,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
   __shared__ V s_stash[S > 1 ? (S - 1) * KR * BLOCK : 1];
 
   const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
+  // wave index and tickets are read as wave-uniform values (scalar registers and
+  // branches): a branch taken for divergent runs both arms under lane masks, and its
+  // waits then cover both arms' loads
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 
   // Masked load of sub-tile `sub` (zeros outside the tile's segment window).
   auto load = [&](T (&dst)[KR][VEC], const TileInfo& t, int sub) {
@@ -97,6 +100,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
         for (int j = 0; j < VEC; ++j) dst[k][j] = T(0);
       }
     }
+    // a partial sub-tile's loads complete here (vmcnt 0), so the whole sub-tiles' loads
+    // into the same registers need no wait of their own (the arms join before them)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
   };
 
   // Per-chunk wave scans carried across the wave's chunks (local type); returns the
@@ -188,7 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     }
     __syncthreads();
     const uint32_t epoch = s_epoch;
-    uint32_t tile = s_tile;
+    uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
     if (tile < g.ntiles) {
       TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
       T x[KR][VEC];
@@ -203,7 +209,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
         if (threadIdx.x == 0)
           s_next = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const uint32_t next = s_next;
+        const uint32_t next = __builtin_amdgcn_readfirstlane(s_next);
         const bool more = next < g.ntiles;
         TileInfo tn = ti;
         T xn[KR][VEC];
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan(const T* in_al, int64_t in_shift
     if (threadIdx.x == 0)
       s_tile = __hip_atomic_fetch_add(&ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
     const uint32_t epoch = s_epoch;
     if (tile >= g.ntiles) break;
     const TileInfo ti = tile_info<VEC, TILE>(g, tile, in_shift);
