@@ -251,6 +251,17 @@ __device__ __forceinline__ A wave_sum(A v) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// ---------------------------------------------------------------- read-only tables
+// Entry i of a table that no kernel writes while it runs, at a wave-uniform index, as a
+// scalar load (constant address space).  A vector load would count in the same in-order
+// counter as the tile's stores, and a wait for it would also wait for every store issued
+// before it.
+template <class T>
+__device__ __forceinline__ T ld_table(const T* p, uint32_t i) {
+  using CP = const __attribute__((address_space(4))) T*;
+  return ((CP)(p))[__builtin_amdgcn_readfirstlane(i)];
+}
+
 // ---------------------------------------------------------------- agent-scope words
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

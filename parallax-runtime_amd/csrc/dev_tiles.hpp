@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dev_common.hpp"
+
 namespace pxk {
 
 struct SegGeom {
@@ -40,16 +42,6 @@ struct TileInfo {
   int64_t w0;           // window start (aligned coordinates, multiple of VEC)
   int64_t out_delta;    // output element index = input element index + out_delta
 };
-
-// Entry i of a table that no kernel writes while it runs, at a wave-uniform index, as a
-// scalar load (constant address space).  A vector load would count in the same in-order
-// counter as the tile's stores, and a wait for it would also wait for every store issued
-// before it.
-template <class T>
-__device__ __forceinline__ T ld_table(const T* p, uint32_t i) {
-  using CP = const __attribute__((address_space(4))) T*;
-  return ((CP)(p))[__builtin_amdgcn_readfirstlane(i)];
-}
 
 template <int VEC, int TILE>
 __device__ __forceinline__ TileInfo tile_info(const SegGeom& g, uint32_t tile, int64_t shift) {

@@ -127,14 +127,15 @@ __device__ __forceinline__ typename RadixKey<T>::U read_key_base(const Ctl* ctl)
 }
 
 // Keys per lane of a whole-array digit pass: 16 for 8-byte keys (1024-thread tiles of 16K
-// keys, 128 KiB of LDS; 18 would need 161 KiB), 22 for 4-byte keys (the most without
-// spills: 24 spills).  Measured: 8-byte keys at 8 per lane 1.85 ms, at 16 per lane
-// 1.51 ms (2^25 f64); 4-byte keys at 16 / 18 / 20 per lane 1.08 / 1.06 / 1.02 ms (2^26
-// u32: fewer tiles, fewer look-backs), and with the LDS peer-mask ranking 20 / 22 per
-// lane 0.886 / 0.869 ms (2^26 u32), 0.228 / 0.219 ms (2^24 i32 in [-1000, 1000]).  The
-// segmented pass keeps 16 (at 20 it spills).
+// keys, 128 KiB of LDS; 18 would need 161 KiB), 24 for 4-byte keys (the most without
+// spills once the ticket and wave index sit in scalar registers: 26 spills).  Measured:
+// 8-byte keys at 8 per lane 1.85 ms, at 16 per lane 1.51 ms (2^25 f64); 4-byte keys at
+// 16 / 18 / 20 per lane 1.08 / 1.06 / 1.02 ms (2^26 u32: fewer tiles, fewer
+// look-backs), with the LDS peer-mask ranking 20 / 22 per lane 0.886 / 0.869 ms, and
+// with scalar ticket and wave index 22 / 24 per lane 0.857 / 0.840 ms.  The segmented
+// pass keeps 16 (at 20 it spills).
 template <class T>
-constexpr int kKpt = sizeof(T) == 4 ? 22 : 16;
+constexpr int kKpt = sizeof(T) == 4 ? 24 : 16;
 template <class T>
 constexpr int kKptSeg = 16;
 
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
   __shared__ __attribute__((aligned(16))) T s_keys[TILE];
 
   const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const uint32_t* bins = ctl->radix_bins + (shift / 8) * kBins;
   const uint64_t lanes_below = (1ull << lane) - 1ull;
   // SEG: the bits of this digit that vary across the batch's long segments; a full tile
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     uint64_t* const mt = reinterpret_cast<uint64_t*>(s_keys) + wid * kBins;
 #pragma unroll
     for (int q = 0; q < kBins / kWave; ++q) mt[q * kWave + lane] = 0ull;
-    const uint32_t tile = s_tile;
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
     if (tile >= ntiles) break;
     // Positions in 32-bit arithmetic (n < 2^32; hipcc lowered a 64-bit form of the
     // tile-length select -- a vector compare, then a scalar select on SCC -- to a select
@@ -550,10 +551,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const T* __restrict__ tin = in;
     T* __restrict__ tout = out;
     if constexpr (SEG) {
-      seg = g.tile_seg[tile];
-      first_tile = g.seg_first[seg];
-      const uint64_t lo64 = g.seg_lo[seg], pk = g.seg_pack[seg];
-      const uint32_t len = static_cast<uint32_t>(g.seg_hi[seg] - lo64);
+      seg = ld_table(g.tile_seg, tile);
+      first_tile = ld_table(g.seg_first, seg);
+      const uint64_t lo64 = ld_table(g.seg_lo, seg), pk = ld_table(g.seg_pack, seg);
+      const uint32_t len = static_cast<uint32_t>(ld_table(g.seg_hi, seg) - lo64);
       tbase = (tile - first_tile) * static_cast<uint32_t>(TILE);
       tend = len - tbase < static_cast<uint32_t>(TILE) ? len : tbase + static_cast<uint32_t>(TILE);
       tin = in + (in_a ? lo64 : pk);
