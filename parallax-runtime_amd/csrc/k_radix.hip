@@ -37,7 +37,9 @@ namespace pxk {
 namespace {
 
 constexpr int kBins = 256;
-constexpr int kLookWindow = 4;  // predecessor tiles read per look-back round trip
+// Predecessor tiles read per look-back round trip (with 24- / 16-key lanes: 2, 3, 4 tiles
+// sort 2^26 u32 in 0.841 / 0.852 / 0.842 ms and 2^25 f64 in 1.281 / 1.281 / 1.295 ms).
+constexpr int kLookWindow = 2;
 
 template <class T>
 struct RadixKey;
