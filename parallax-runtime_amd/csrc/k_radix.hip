@@ -134,13 +134,13 @@ __device__ __forceinline__ typename RadixKey<T>::U read_key_base(const Ctl* ctl)
 // 8-byte keys at 8 per lane 1.85 ms, at 16 per lane 1.51 ms (2^25 f64); 4-byte keys at
 // 16 / 18 / 20 per lane 1.08 / 1.06 / 1.02 ms (2^26 u32: fewer tiles, fewer
 // look-backs), with the LDS peer-mask ranking 20 / 22 per lane 0.886 / 0.869 ms, and
-// with scalar ticket and wave index 22 / 24 per lane 0.857 / 0.840 ms.  The segmented
-// pass: 20 for 4-byte keys (24 spills; served mixed passes 4.13 -> 4.06 ms against 16),
-// 16 for 8-byte keys.
+// with scalar ticket and wave index 22 / 24 per lane 0.857 / 0.840 ms (26 spills).  The
+// segmented pass: 24 for 4-byte keys (served mixed passes 4.13 -> 4.06 ms at 20 against
+// 16; served sort batches 2.81 -> 2.78 ms at 24 against 20), 16 for 8-byte keys.
 template <class T>
 constexpr int kKpt = sizeof(T) == 4 ? 24 : 16;
 template <class T>
-constexpr int kKptSeg = sizeof(T) == 4 ? 20 : 16;
+constexpr int kKptSeg = sizeof(T) == 4 ? 24 : 16;
 
 __device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint32_t state, uint32_t v) {
   return (static_cast<uint64_t>((epoch << 2) | state) << 32) | v;
