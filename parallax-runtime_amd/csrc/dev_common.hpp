@@ -259,7 +259,7 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 template <class T>
 __device__ __forceinline__ T ld_table(const T* p, uint32_t i) {
   using CP = const __attribute__((address_space(4))) T*;
-  return ((CP)(p))[__builtin_amdgcn_readfirstlane(i)];
+  return ((CP)(p))[static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(i))];
 }
 
 // ---------------------------------------------------------------- agent-scope words

@@ -151,7 +151,9 @@ __global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ in_al, in
 
 // Segmented reduce, pass 1: tile partials.  Tiles are independent (static grid-stride
 // assignment, no atomics); a segment held by one tile writes its result directly, the
-// others leave one partial per tile for pass 2.
+// others leave one partial per tile for pass 2.  Software-pipelined: a block looks up its
+// next tile and issues that tile's loads before it adds up the current one, so the
+// table lookups and the memory latency of the next tile overlap this one's work.
 template <class T, int K>
 __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al, int64_t shift,
                                                     SegGeom g,
@@ -164,16 +166,53 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
   const V* vin = reinterpret_cast<const V*>(in_al);
   __shared__ A s_w[4];
   const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 
-  for (uint32_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
-    const TileInfo ti = tile_info<VEC, TILE>(g, tile, shift);
-    // four accumulator chains (vector k adds into chain k % 4), folded in a fixed order
-    A acc[4] = {acc_zero<A>(), acc_zero<A>(), acc_zero<A>(), acc_zero<A>()};
-    if (ti.w0 >= ti.lo && ti.w0 + TILE <= ti.hi) {  // whole tile inside the segment
-      V v[K];
+  // Tile-relative bounds in 32 bits: the segment covers [lo_r, hi_r) of the tile window
+  // (clamped to [0, TILE]); `nvec` of the window's vectors overlap it.  A tile window starts
+  // at or after its segment's aligned start (w0 >= lo rounded down), so only the first
+  // tile can start before `lo`.  Uniform 64-bit differences are split into 32-bit halves:
+  // gfx9 compares signed 64-bit values only in vector registers.
+  struct Rel {
+    int lo_r, hi_r;
+    uint32_t nvec;
+  };
+  auto rel = [&](const TileInfo& t) {
+    auto clamp_tile = [](int64_t d) -> int {  // d clamped to [0, TILE]
+      const int32_t dh = static_cast<int32_t>(static_cast<uint64_t>(d) >> 32);
+      const uint32_t dl = static_cast<uint32_t>(d);
+      return dh < 0 ? 0 : (dh > 0 ? TILE : (dl < static_cast<uint32_t>(TILE) ? static_cast<int>(dl) : TILE));
+    };
+    Rel r;
+    r.lo_r = clamp_tile(t.lo - t.w0);
+    r.hi_r = clamp_tile(t.hi - t.w0);
+    r.nvec = static_cast<uint32_t>(r.hi_r + VEC - 1) / VEC;
+    return r;
+  };
+  // K vector loads per thread for every tile, so a wait for the current tile counts the
+  // next tile's K loads out.  They are buffer loads over the tile window's vectors that
+  // overlap its segment: a vector outside them reads as zero without touching memory (an
+  // empty segment's tile reads nothing), and `add` masks the positions outside the
+  // segment.
+  auto load = [&](const TileInfo& t, const Rel& r, V (&v)[K]) {
+    // the base address in scalar registers: readfirstlane returns int, so each half is
+    // taken as uint32_t before widening (a sign-extended low half would set every bit of
+    // the high half)
+    const uint64_t a64 = reinterpret_cast<uint64_t>(vin + t.w0 / VEC);
+    const uint32_t ahi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a64 >> 32)));
+    const uint32_t alo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a64)));
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<void*>((static_cast<uint64_t>(ahi) << 32) | alo), 0,
+        static_cast<int>(__builtin_amdgcn_readfirstlane(r.nvec * 16u)), 0x00020000);
 #pragma unroll
-      for (int k = 0; k < K; ++k) v[k] = ld_stream(vin + ti.w0 / VEC + k * 256 + threadIdx.x);
+    for (int k = 0; k < K; ++k)
+      v[k] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(
+                                       rs, static_cast<int>((k * 256u + threadIdx.x) * 16u), 0, 2));
+  };
+  // four accumulator chains (vector k adds into chain k % 4), folded in a fixed order
+  auto add = [&](const Rel& r, const V (&v)[K]) {
+    A acc[4] = {acc_zero<A>(), acc_zero<A>(), acc_zero<A>(), acc_zero<A>()};
+    if (r.lo_r == 0 && r.hi_r == TILE) {  // whole tile inside the segment
 #pragma unroll
       for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -181,26 +220,49 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
     } else {
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const int64_t p0 = ti.w0 + (static_cast<int64_t>(k) * 256 + threadIdx.x) * VEC;
-        if (p0 < ti.hi && p0 + VEC > ti.lo) {
-          const V v = vin[p0 / VEC];
+        const int p0 = (k * 256 + static_cast<int>(threadIdx.x)) * VEC;
 #pragma unroll
-          for (int j = 0; j < VEC; ++j)
-            if (p0 + j >= ti.lo && p0 + j < ti.hi) acc[k & 3] = acc_add(acc[k & 3], to_acc(v.v[j]));
-        }
+        for (int j = 0; j < VEC; ++j)
+          if (p0 + j >= r.lo_r && p0 + j < r.hi_r) acc[k & 3] = acc_add(acc[k & 3], to_acc(v[k].v[j]));
       }
     }
-    const A w = wave_sum(acc_add(acc_add(acc[0], acc[1]), acc_add(acc[2], acc[3])));
+    return acc_add(acc_add(acc[0], acc[1]), acc_add(acc[2], acc[3]));
+  };
+
+  uint32_t tile = blockIdx.x;
+  if (tile >= g.ntiles) return;
+  TileInfo ti = tile_info<VEC, TILE>(g, tile, shift);
+  Rel ri = rel(ti);
+  // One tile: issue the next tile's loads into `nxt`, add up `cur`, write the result.
+  // The two register sets alternate, so no copy waits for the next tile's loads.
+  auto step = [&](const V (&cur)[K], V (&nxt)[K]) {
+    const uint32_t next = tile + gridDim.x;
+    const bool more = next < g.ntiles;
+    // the last tile loads itself again: every step issues the same loads
+    const TileInfo tn = more ? tile_info<VEC, TILE>(g, next, shift) : ti;
+    const Rel rn = rel(tn);
+    load(tn, rn, nxt);
+    // the next tile's loads stay ahead of this tile's adds
+    __builtin_amdgcn_sched_barrier(0);
+    const A w = wave_sum(add(ri, cur));
     if (lane == 0) s_w[wid] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
       const A tot = acc_add(acc_add(s_w[0], s_w[1]), acc_add(s_w[2], s_w[3]));
       if (ti.first_tile == ti.last_tile)
-        out[g.seg_out ? g.seg_out[ti.seg] : ti.seg] = from_acc<T>(tot);
+        out[g.seg_out ? ld_table(g.seg_out, ti.seg) : ti.seg] = from_acc<T>(tot);
       else
         partials[tile] = tot;
     }
     __syncthreads();
+    ti = tn;
+    ri = rn;
+    tile = next;
+    return more;
+  };
+  V va[K], vb[K];
+  load(ti, ri, va);
+  while (step(va, vb) && step(vb, va)) {
   }
 }
 

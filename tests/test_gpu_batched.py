@@ -58,6 +58,39 @@ def test_seg_reduce(gpu_ctx, dt):
         assert np.array_equal(got, pxo.seg_reduce(a, off))
 
 
+@pytest.mark.parametrize("dt", [PX_F32, PX_U64])
+def test_seg_reduce_input_above_bit31(gpu_ctx, dt):
+    """Segmented reduce of an input whose addresses have bit 31 of the low word set (the
+    tile loads build a buffer base address from 32-bit halves; a sign-extended low half
+    would corrupt it).  A 2 GiB + 128 MiB allocation always holds 64 MiB of such
+    addresses in a row: at its start, or from the next low word 0x80000000."""
+    a, off = seg_input(dt, 40 + dt)
+    assert a.nbytes <= 48 << 20
+    big = DeviceBuffer((1 << 31) + (128 << 20))
+    low = big.ptr & 0xFFFFFFFF
+    if low & 0x80000000 and (1 << 32) - low >= (64 << 20):
+        shift = 0
+    else:
+        shift = ((1 << 31) - low) % (1 << 32)
+    shift += a.itemsize  # not 16-byte aligned: the head and tail vectors are partial
+    first, last = big.addr(shift), big.addr(shift) + a.nbytes - 1
+    assert shift + a.nbytes <= big.nbytes
+    assert first & 0x80000000 and last & 0x80000000 and first >> 32 == last >> 32
+    big.upload(a, shift)
+    out = DeviceBuffer(8 * (off.size - 1))
+    check(lib.pxb_seg_reduce(gpu_ctx.h, dt, big.addr(shift), u64p(off), off.size - 1, out.ptr), "seg_reduce")
+    gpu_ctx.sync()
+    got = out.download(a.dtype, off.size - 1)
+    if dt in FLOATS:
+        hi, ab = pxo.seg_sum_exact(a, off)
+        lens = np.diff(off.astype(np.int64))
+        for s in range(off.size - 1):
+            assert abs(float(got[s]) - hi[s]) <= bound(dt, lens[s]) * ab[s] + 1e-300, s
+    else:
+        assert np.array_equal(got, pxo.seg_reduce(a, off))
+    big.free()
+
+
 @pytest.mark.parametrize("dt", ALL)
 def test_seg_scan(gpu_ctx, dt):
     a, off = seg_input(dt, 20 + dt)
