@@ -262,6 +262,28 @@ __device__ __forceinline__ T ld_table(const T* p, uint32_t i) {
   return ((CP)(p))[static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(i))];
 }
 
+// ---------------------------------------------------------------- buffer resources
+// A raw buffer over [p, p + nbytes) for a wave-uniform p and nbytes: loads past nbytes
+// return zero without touching memory, stores there are dropped.  readfirstlane returns
+// int, so each half of the address is taken as uint32_t before widening (a sign-extended
+// low half would set every bit of the high half).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void* p, uint32_t nbytes) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32)));
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a)));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>((static_cast<uint64_t>(hi) << 32) | lo), 0,
+                                           static_cast<int>(__builtin_amdgcn_readfirstlane(nbytes)), 0x00020000);
+}
+
+// A wave-uniform 64-bit difference clamped to [0, m] (m < 2^31) with 32-bit compares on
+// its halves: gfx9 compares signed 64-bit values only in vector registers, whose
+// temporaries can collide with the destinations of loads in flight.
+__device__ __forceinline__ int clamp_span(int64_t d, int m) {
+  const int32_t dh = static_cast<int32_t>(static_cast<uint64_t>(d) >> 32);
+  const uint32_t dl = static_cast<uint32_t>(d);
+  return dh < 0 ? 0 : (dh > 0 ? m : (dl < static_cast<uint32_t>(m) ? static_cast<int>(dl) : m));
+}
+
 // ---------------------------------------------------------------- agent-scope words
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

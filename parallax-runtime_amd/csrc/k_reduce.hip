@@ -178,14 +178,9 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
     uint32_t nvec;
   };
   auto rel = [&](const TileInfo& t) {
-    auto clamp_tile = [](int64_t d) -> int {  // d clamped to [0, TILE]
-      const int32_t dh = static_cast<int32_t>(static_cast<uint64_t>(d) >> 32);
-      const uint32_t dl = static_cast<uint32_t>(d);
-      return dh < 0 ? 0 : (dh > 0 ? TILE : (dl < static_cast<uint32_t>(TILE) ? static_cast<int>(dl) : TILE));
-    };
     Rel r;
-    r.lo_r = clamp_tile(t.lo - t.w0);
-    r.hi_r = clamp_tile(t.hi - t.w0);
+    r.lo_r = clamp_span(t.lo - t.w0, TILE);
+    r.hi_r = clamp_span(t.hi - t.w0, TILE);
     r.nvec = static_cast<uint32_t>(r.hi_r + VEC - 1) / VEC;
     return r;
   };
@@ -195,15 +190,7 @@ __global__ __launch_bounds__(256) void k_seg_reduce(const T* __restrict__ in_al,
   // empty segment's tile reads nothing), and `add` masks the positions outside the
   // segment.
   auto load = [&](const TileInfo& t, const Rel& r, V (&v)[K]) {
-    // the base address in scalar registers: readfirstlane returns int, so each half is
-    // taken as uint32_t before widening (a sign-extended low half would set every bit of
-    // the high half)
-    const uint64_t a64 = reinterpret_cast<uint64_t>(vin + t.w0 / VEC);
-    const uint32_t ahi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a64 >> 32)));
-    const uint32_t alo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a64)));
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        reinterpret_cast<void*>((static_cast<uint64_t>(ahi) << 32) | alo), 0,
-        static_cast<int>(__builtin_amdgcn_readfirstlane(r.nvec * 16u)), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(vin + t.w0 / VEC, r.nvec * 16u);
 #pragma unroll
     for (int k = 0; k < K; ++k)
       v[k] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(
