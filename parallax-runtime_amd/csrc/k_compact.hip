@@ -132,42 +132,37 @@ __global__ __launch_bounds__(BLOCK) void k_compact(
   auto load = [&](Tile& u, const TileInfo& ti) {
     u.w0 = ti.w0;
     u.lo = ti.lo;
+    // window-relative bounds in 32 bits (uniform 64-bit differences split into halves)
     const int64_t dl = ti.lo - u.w0, dh = ti.hi - u.w0;
-    u.lo_r = dl < 0 ? -1 : (dl > TILE ? TILE : static_cast<int>(dl));
-    u.hi_r = dh > TILE ? TILE : (dh < 0 ? 0 : static_cast<int>(dh));
+    u.lo_r = static_cast<int32_t>(static_cast<uint64_t>(dl) >> 32) < 0 ? -1 : clamp_span(dl, TILE);
+    u.hi_r = clamp_span(dh, TILE);
     u.vstart_r = u.lo_r > 0 ? u.lo_r : 0;
     const bool full = u.lo_r <= 0 && u.hi_r == TILE;
-    const V* tp = vin + u.w0 / VEC;
 
-    // ---- raw loads (chunks wholly outside the segment are not read) and valid bits
-    u.valid = 0;
+    // ---- raw loads and valid bits.  K buffer loads per thread for every tile, over the
+    // window's vectors up to the segment's end (past them they read zero without touching
+    // memory), so a wait for this tile counts out the loads issued after it; a partial
+    // tile differs only in its valid bits (vector arithmetic, no memory branch).
+    const uint32_t nvec = static_cast<uint32_t>(u.hi_r + VEC - 1) / VEC;
+    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(vin + u.w0 / VEC, nvec * 16u);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const V v = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(
+                                            rs, base_rel(k) / VEC * 16, 0, 2));
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = v.v[j];
+    }
     if (full) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const V v = ld_stream(tp + base_rel(k) / VEC);
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = v.v[j];
-      }
       u.valid = (E == 32) ? 0xffffffffu : ((1u << E) - 1u);
     } else {
+      u.valid = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const int r0 = base_rel(k);
-        if (r0 < u.hi_r && r0 + VEC > u.lo_r) {
-          const V v = tp[r0 / VEC];
-#pragma unroll
-          for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = v.v[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < VEC; ++j) u.x[k * VEC + j] = T(0);
-        }
 #pragma unroll
         for (int j = 0; j < VEC; ++j)
           if (r0 + j >= u.lo_r && r0 + j < u.hi_r) u.valid |= 1u << (k * VEC + j);
       }
-      // a partial tile's loads complete here (vmcnt 0), so the full tiles' loads into
-      // the same registers need no wait of their own (the arms join before them)
-      __builtin_amdgcn_s_waitcnt(0x0F70);
     }
   };
 
