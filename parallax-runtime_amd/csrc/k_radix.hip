@@ -385,13 +385,14 @@ constexpr int kSegBlockLarge = 1024;
 
 // Tile t of the long segments covers [tbase, tend) of its segment (positions relative
 // to the segment's first key: a segment holds fewer than 2^32 keys, the batch may not).
+// `tile` is wave-uniform; the tables are read by scalar loads.
 template <class T>
 __device__ __forceinline__ void seg_tile(const RadixSegs& g, uint32_t tile, uint32_t& seg,
                                          uint32_t& tbase, uint32_t& tend) {
   const uint32_t TILE = g.tile;
-  seg = g.tile_seg[tile];
-  const uint32_t len = static_cast<uint32_t>(g.seg_hi[seg] - g.seg_lo[seg]);
-  tbase = (tile - g.seg_first[seg]) * TILE;
+  seg = ld_table(g.tile_seg, tile);
+  const uint32_t len = static_cast<uint32_t>(ld_table(g.seg_hi, seg) - ld_table(g.seg_lo, seg));
+  tbase = (tile - ld_table(g.seg_first, seg)) * TILE;
   tend = len - tbase < TILE ? len : tbase + TILE;
 }
 
@@ -447,8 +448,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
           uint32_t sg, tb, te;
           seg_tile<T>(g, t, sg, tb, te);
-          const T* ti = in + (in_a ? g.seg_lo[sg] : g.seg_pack[sg]);
-          T* to = out + (out_a ? g.seg_lo[sg] : g.seg_pack[sg]);
+          const T* ti = in + (in_a ? ld_table(g.seg_lo, sg) : ld_table(g.seg_pack, sg));
+          T* to = out + (out_a ? ld_table(g.seg_lo, sg) : ld_table(g.seg_pack, sg));
           for (uint32_t p = tb + threadIdx.x; p < te; p += BLOCK) to[p] = ti[p];
         }
         return;
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(256) void k_segradix_hist(const T* data_all, RadixS
       if (cur != 0xffffffffu) flush(cur);
       cur = seg;
     }
-    const T* data = data_all + g.seg_lo[seg];  // positions are relative to the segment
+    const T* data = data_all + ld_table(g.seg_lo, seg);  // positions are relative to the segment
     const uint32_t mis = static_cast<uint32_t>(((reinterpret_cast<uintptr_t>(data + tbase)) & 15u) / sizeof(T));
     uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
     if (head > tend - tbase) head = tend - tbase;
@@ -1003,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_segradix_min(const T* data_all, RadixSe
   for (uint32_t t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
     uint32_t sg, tb, te;
     seg_tile<T>(g, t, sg, tb, te);
-    const T* data = data_all + g.seg_lo[sg];
+    const T* data = data_all + ld_table(g.seg_lo, sg);
     // 16-byte loads for the tile's aligned body, single keys for its head and tail
     const uint32_t mis = static_cast<uint32_t>(((reinterpret_cast<uintptr_t>(data + tb)) & 15u) / sizeof(T));
     uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
