@@ -1001,8 +1001,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (every GPU box here has one GPU; the sharded "
-              "design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: measures one GPU only (multi-GPU runs are not supported: the sharded "
+              "form is a design, not code); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "kernels":
         kernels_once(args)
